@@ -1,0 +1,183 @@
+"""Benchmark of the PBF step on one MI355X.
+
+A step is one substep of the 1,048,576-particle dam break (Fluid::compute:
+predict + hash, sort, reorder, 4 Jacobi iterations of lambda / delta p with
+collisions, update, vorticity, XSPH) with the particles already resident in
+HBM.  Prints one JSON line with the throughput, the roofline of the dominant
+kernel (per-launch algorithmic bytes / HIP-event-timed launch duration) and
+the CPU oracle timed on this host as the baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pbd-simulation-framework-cuda_amd"))
+
+METRIC = "particle-steps/s @1M PBF (4 iters); %HBM roofline; batched vs plain; adj/fwd"
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes(stage, n, p, table_size, passes):
+    """Algorithmic HBM bytes of one launch of a stage (each datum read / written once)."""
+    gates_fn = p.active_find_neighbours
+    per = {
+        # R x, v, f_ext (48) ; W cell id (4)
+        "predict_keys": 52,
+        # R cell id + iota-free value pass structure: histogram R 4, then per pass R key (+ val) W key + val
+        "sort": 4 + 12 + (passes - 1) * 16,
+        # R sorted key + id (8), gather x, v, f_ext (48), W q, qx, v (48), colours when not recomputed (32)
+        "reorder": 104 + (0 if gates_fn else 32),
+        # R q (16), cell id (4); W lambda (4) [+ density, counts on the first pass]
+        "lambda": 24,
+        # R q, qx (32), lambda (4), cell id (4); W q', qx' (32)
+        "delta": 72,
+        "collide": 68,
+        # R q, qx ; W qx, v
+        "update": 64,
+        "count": 12,
+        # R q, v (32), cell id (4); W omega (16)
+        "omega": 52,
+        # R q, qx, v, omega (64), cell id, count (8); W x, p*, v, colour, f_ext (80)
+        "final": 152,
+    }
+    if stage == "cell_reset":
+        return table_size * 4  # one of the two fill launches
+    return per.get(stage, 0) * n
+
+
+def cpu_baseline(p, steps):
+    """The CPU oracle (FP64 restatement of the same step, OpenMP) timed on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pbdfluid as F
+    import pyoracle as O
+
+    pos, vel, col = F.generate_box(p)
+    st = O.State(pos, vel, col)
+    n = pos.shape[0]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        O.step(p, st, want_outputs=False)
+    dt = time.perf_counter() - t0
+    return {
+        "value": n * steps / dt,
+        "unit": "particle-steps/s",
+        "cores": O.threads(),
+        "kind": "port",
+        "sample": f"{steps} substep(s) of the same {n}-particle dam break on the FP64 C oracle "
+                  f"(OpenMP, {O.threads()} threads), {dt:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="dam_break_1m")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the step runs on a single "
+              f"MI355X; the multi-GPU decomposition is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import pbdfluid as F
+
+    p = F.load_params(F.config_path(args.config))
+    p.box_spawn_in_the_beginning = 1
+    ctx = F.Context(p)
+    n = ctx.n
+    table = int(F.capi.lib().pbd_morton_encode(p.grid_dims[0] - 1, p.grid_dims[1] - 1, p.grid_dims[2] - 1)) + 1
+    passes = max(1, (table - 1).bit_length() + 7 >> 3)
+
+    for _ in range(args.warmup):
+        ctx.compute()
+    ctx.sync()
+
+    # timed region: K substeps, per-kernel HIP events on the context's stream
+    ctx.set_timing(True)
+    ctx.reset_timings()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.compute()
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    timings = ctx.timings()
+    ctx.set_timing(False)
+    value = n * args.steps / elapsed
+
+    # the same K substeps replayed as a captured hipGraph (no events)
+    graph_value = None
+    if not args.no_graph:
+        q = p.copy()
+        q.substeps = 1  # one graph replay = one substep
+        ctx.set_params(q)
+        ctx.step(2, graph=True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(args.steps, graph=True)
+        ctx.sync()
+        graph_value = n * args.steps / (time.perf_counter() - t0)
+    bad = ctx.check_finite()
+    ctx.close()
+
+    stages = {k: {"ms_total": v[0], "launches": v[1], "us_per_launch": (1e3 * v[0] / v[1]) if v[1] else 0.0}
+              for k, v in timings.items() if v[1]}
+    dom = max(stages, key=lambda k: stages[k]["ms_total"])
+    avg_s = stages[dom]["ms_total"] / stages[dom]["launches"] / 1e3
+    bytes_launch = stage_bytes(dom, n, p, table, passes)
+    achieved = bytes_launch / avg_s / 1e9
+    for k, v in stages.items():
+        b = stage_bytes(k, n, p, table, passes)
+        v["algorithmic_GBps"] = (b / (v["us_per_launch"] * 1e-6) / 1e9) if v["us_per_launch"] else 0.0
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.config}: {n}-particle PBF dam break (64x128x128 lattice, spacing 0.05, "
+                        "h = cell = 0.1, 4 Jacobi iterations, 1 stabilization pass, dt = 1/120 per step)",
+            "particles": n,
+            "solver_iterations": p.solver_iterations,
+            "kernel_variant": "textbook" if p.kernel_variant == 1 else "reference",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "graph_value": graph_value,
+        "steps_per_s": args.steps / elapsed,
+        "stages": stages,
+        "nonfinite_particles": bad,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(p, args.cpu_steps)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,259 @@
+/*
+ * pbd.h - C-ABI of the MI355X-native position-based-fluids (PBF) step.
+ *
+ * This is the drop-in boundary for the per-frame simulation step of
+ * niklasandersson/PBD-Simulation-Framework-CUDA.  In the reference the step is
+ * reached through
+ *   - Cuda_Computable::compute()           PBD-Simulation-Framework-CUDA/src/cuda/Cuda_Computable.h:8-18
+ *   - Fluid::Fluid / ~Fluid / compute()     PBD-Simulation-Framework-CUDA/src/app/simulation/Fluid.cu:4-97
+ *   - Simulation::step()                    PBD-Simulation-Framework-CUDA/src/app/Simulation.cu:16-19
+ *   - the argument-less cudaCall*() stage functions declared in
+ *     PBD-Simulation-Framework-CUDA/src/app/simulation/kernel/Kernels.h:18-43
+ * with parameters pulled from the Config singleton (src/parser/Config.h:21-32)
+ * and particle buffers owned by OpenGL (src/app/rendering/Particles.cc:4-147).
+ * Here the library owns device memory, parameters travel in a POD struct,
+ * and every call returns a status code (0 = OK); no exception crosses the ABI.
+ *
+ * Particle-buffer layout (kept from the reference, kernel/Globals.h:5-13,129-132):
+ * four float4 arrays positions4 / predictedPositions4 / velocities4 / colors4,
+ * particle i at element i (the reference's texel (i mod W, i div W) of a
+ * W x W RGBA32F texture is the same linear float4 array).  After each step the
+ * particles are stored in the step's sorted (cell-id, previous-index) order, as
+ * in the reference, which reorders its surfaces every frame (SortReorder.h:86-129).
+ *
+ * Threading: one context is bound to one HIP stream and is not re-entrant;
+ * callers serialize, like the reference's single simulation thread.
+ */
+#ifndef PBD_PBD_H
+#define PBD_PBD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBD_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  PBD_OK = 0,
+  PBD_ERR_INVALID = 1,     /* bad argument or parameter out of range          */
+  PBD_ERR_CAPACITY = 2,    /* particle capacity exceeded (reference ignores it, Communication.h:631,642) */
+  PBD_ERR_HIP = 3,         /* a HIP runtime call failed                         */
+  PBD_ERR_CONFIG = 4,      /* config.txt missing, malformed or key not found    */
+  PBD_ERR_NONFINITE = 5,   /* NaN/Inf found by pbd_check_finite                 */
+  PBD_ERR_UNSUPPORTED = 6, /* feature not available in this build             */
+  PBD_ERR_NOMEM = 7        /* host or device allocation failed                  */
+};
+
+/* ---- kernel variants (Density.kernelVariant) ---------------------------- */
+enum {
+  PBD_KERNEL_REFERENCE = 0, /* literal reference formulas, kernel/Density.h:14-42 */
+  PBD_KERNEL_TEXTBOOK = 1   /* Macklin & Mueller 2013 PBF with Mueller 2003 kernels */
+};
+
+/*
+ * pbd_params - POD mirror of the reference's config.txt key set.
+ * Replaces SimulationParameters + initializeFrame()'s Config reads
+ * (kernel/Globals.h:39-79, kernel/Initialize.h:20-71).  Each field names the
+ * config key it mirrors; keys marked "new" are additions with defaults that
+ * reproduce the reference.
+ */
+typedef struct pbd_params {
+  /* Application.Cuda */
+  uint32_t threads_per_block;          /* Application.Cuda.threadsPerBlock (launch block size) */
+  /* Application.OpenGL.Textures */
+  uint32_t texture_width;              /* Application.OpenGL.Textures.width; capacity = width^2 */
+  /* Application.Simulation.Active.* (stage gates, Fluid.cu:21-95) */
+  int32_t active_apply_forces;
+  int32_t active_find_contacts;
+  int32_t active_find_neighbours;
+  int32_t active_compute_lambda;
+  int32_t active_compute_delta_positions;
+  int32_t active_apply_delta_positions;
+  int32_t active_update_positions;
+  int32_t active_compute_omegas;
+  int32_t active_compute_vorticity;
+  int32_t active_compute_viscosity;
+  /* Application.Simulation.Collision */
+  int32_t collision_type;              /* 0 = per-contact push, 1 = batched pair solver */
+  uint32_t kernel_width_neighbours;    /* neighbour stencil half-width in cells */
+  uint32_t max_collision_batches;
+  float stiffness;
+  /* Application.Simulation.Density */
+  float k_scorr;
+  float kernel_width_spiky;            /* multiple of the particle diameter */
+  int32_t n_scorr;
+  float q_scorr;                       /* multiple of the spiky kernel width */
+  float rest_density;
+  float relaxation;                    /* new: epsilon in lambda (reference hard-codes 1e-4) */
+  int32_t rest_density_from_lattice;   /* new: rest density from the spawn lattice */
+  int32_t kernel_variant;              /* new: PBD_KERNEL_REFERENCE / PBD_KERNEL_TEXTBOOK */
+  /* Application.Simulation.Enclosure */
+  float enclosure_position_damping;
+  float enclosure_velocity_damping;
+  float bounds_min[3];                 /* Enclosure.X|Y|Z.min */
+  float bounds_max[3];                 /* Enclosure.X|Y|Z.max */
+  /* Application.Simulation.Forces */
+  float delta_t;                       /* frame time step */
+  float gravity;
+  uint32_t substeps;                   /* new: substeps per frame (dt = delta_t / substeps) */
+  /* Application.Simulation.Iterations */
+  uint32_t solver_iterations;
+  uint32_t stabilization_iterations;
+  /* Application.Simulation.Particles */
+  uint32_t max_neighbours_per_particle;
+  float particle_radius;
+  uint32_t box_width;                  /* Particles.Box.width  (lattice count along x) */
+  uint32_t box_height;                 /* Particles.Box.height (along y) */
+  uint32_t box_depth;                  /* Particles.Box.depth  (along z) */
+  float box_spacing;
+  int32_t box_spawn_in_the_beginning;
+  float box_origin[3];                 /* new: reference spawns at (30, 32, 30), Particles.cc:197,205 */
+  float box_jitter;                    /* new: uniform jitter half-width per axis */
+  uint32_t box_seed_lo;                /* new: 64-bit seed of the jitter RNG, low word */
+  uint32_t box_seed_hi;                /* new: high word */
+  /* Application.Simulation.Viscosity */
+  float c_viscosity;
+  float kernel_width_poly;             /* absolute */
+  /* Application.Simulation.Vorticity */
+  float e_vorticity;
+  /* Application.Simulation.Grid (new) */
+  float grid_cell_size;                /* <= 0 selects the particle diameter (reference: 1.0 = 2r) */
+  float grid_origin[3];
+  uint32_t grid_dims[3];               /* cells per axis, each in [1, 1024]; reference 128^3 */
+} pbd_params;
+
+/* Device pointers of one context (valid until the next pbd_step / add / clear). */
+typedef struct pbd_buffers {
+  void* positions4;           /* float4[capacity] */
+  void* predicted_positions4; /* float4[capacity] */
+  void* velocities4;          /* float4[capacity] */
+  void* colors4;              /* float4[capacity] */
+  void* external_forces4;     /* float4[capacity]: vorticity force carried to the next step */
+  uint32_t num_particles;
+  uint32_t capacity;
+} pbd_buffers;
+
+/* Buffer selectors for pbd_download / pbd_upload. */
+enum {
+  PBD_BUF_POSITIONS = 0,        /* float4, positions4 */
+  PBD_BUF_PREDICTED = 1,        /* float4, predictedPositions4 */
+  PBD_BUF_VELOCITIES = 2,       /* float4, velocities4 */
+  PBD_BUF_COLORS = 3,           /* float4, colors4 */
+  PBD_BUF_EXTERNAL_FORCES = 4,  /* float4, d_externalForces */
+  PBD_BUF_DENSITIES = 5,        /* float,  density of the first solver iteration */
+  PBD_BUF_LAMBDAS = 6,          /* float,  d_lambdas (last solver iteration) */
+  PBD_BUF_CELL_IDS = 7,         /* uint32, sorted cell ids (d_cellIds_out) */
+  PBD_BUF_PARTICLE_IDS = 8,     /* uint32, sorted previous indices (d_particleIds_out) */
+  PBD_BUF_CELL_STARTS = 9,      /* uint32[table], d_cellStarts, empty = 0xFFFFFFFF */
+  PBD_BUF_CELL_ENDINGS = 10,    /* uint32[table], d_cellEndings, empty = N */
+  PBD_BUF_NEIGHBOUR_COUNTERS = 11, /* uint32, d_neighbourCounters */
+  PBD_BUF_CONTACT_COUNTERS = 12,   /* uint32, d_contactCounters */
+  PBD_BUF_OMEGAS = 13,          /* float4 (w unused), d_omegas */
+  PBD_BUF_DELTA_POSITIONS = 14, /* float4, d_deltaPositions of the last iteration */
+  PBD_BUF_COUNT = 15
+};
+
+/* ---- parameters / config.txt (host only; no GPU needed) ---------------- */
+
+/* sizeof(pbd_params), for bindings that mirror the struct. */
+size_t pbd_params_size(void);
+/* Fill with the values of the reference's shipped config.txt. */
+int pbd_params_default(pbd_params* out);
+/* Parse a config.txt (reference grammar: scopes, key = value;, arrays,
+ * strings, // and block comments, #include, #define/#undef, true/false)
+ * and fill *out; keys absent from the file keep their pbd_params_default value.
+ * Replaces Config::getInstance()/getValue (src/parser/Config.h:21-32,107-111). */
+int pbd_params_load(const char* path, pbd_params* out, char* err, size_t err_len);
+/* Like pbd_params_load but starts from *inout and only overwrites the keys
+ * present in the file (no missing-key error). */
+int pbd_params_merge(const char* path, pbd_params* inout, char* err, size_t err_len);
+/* Set one key by dotted path with the reference console's range checks
+ * (kernel/Communication.h:440-624). */
+int pbd_params_set(pbd_params* p, const char* key, const char* value, char* err, size_t err_len);
+/* Read one key by dotted path as text. */
+int pbd_params_get(const pbd_params* p, const char* key, char* buf, size_t buf_len);
+/* Write the parameters as config.txt text (console command 'w',
+ * src/parser/ConfigParser.h:241-247).  Returns the needed size in *needed. */
+int pbd_params_write(const pbd_params* p, char* buf, size_t buf_len, size_t* needed);
+/* Generate the spawn lattice of Particles.Box (src/app/rendering/Particles.cc:189-222):
+ * point (i,j,k) at box_origin + spacing*(i,j,k) + jitter, i over width (x),
+ * j over height (y), k over depth (z), index (i*height + j)*depth + k;
+ * v = 0; colour from the seeded RNG with w = 1.  Any output pointer may be NULL. */
+int pbd_generate_box(const pbd_params* p, float* pos4, float* vel4, float* col4,
+                     uint32_t capacity, uint32_t* n_out);
+/* Rest density of the spawn lattice under the selected kernel variant. */
+int pbd_lattice_rest_density(const pbd_params* p, double* out);
+/* Morton code of integer cell coordinates (kernel/Hash.h:10-29; x at bit 3b+2,
+ * y at 3b+1, z at 3b), and the cell id / cell coordinates of a position with
+ * the FP32 arithmetic the step uses (same code as the GPU path). */
+uint32_t pbd_morton_encode(uint32_t x, uint32_t y, uint32_t z);
+int pbd_cell_key(const pbd_params* p, const float pos[3], uint32_t* key, uint32_t ijk[3]);
+
+/* ---- context ------------------------------------------------------------ */
+
+typedef struct pbd_ctx pbd_ctx;
+
+/* Create a context on the current HIP device.  max_particles = 0 selects
+ * texture_width^2 (Particles.cc:11-12).  Replaces cudaInitializeKernels
+ * (kernel/Initialize.h:122-130) plus the GL buffer allocation of Particles. */
+int pbd_create(const pbd_params* p, uint32_t max_particles, pbd_ctx** out);
+/* Replace the parameters; applied at the next step (initializeFrame,
+ * kernel/Initialize.h:20-86). */
+int pbd_set_params(pbd_ctx* ctx, const pbd_params* p);
+int pbd_get_params(pbd_ctx* ctx, pbd_params* out);
+/* Append n particles (addParticlesCallback, kernel/Communication.h:637-655).
+ * Returns PBD_ERR_CAPACITY instead of silently ignoring an overflow.
+ * vel4 / col4 may be NULL (zero velocity, white). */
+int pbd_add_particles(pbd_ctx* ctx, uint32_t n, const float* pos4, const float* vel4,
+                      const float* col4);
+/* One particle at pos with velocity 50*dir, red (addParticleCallback,
+ * kernel/Communication.h:22-38,626-635). */
+int pbd_add_particle(pbd_ctx* ctx, const float pos[3], const float dir[3]);
+/* Spawn the Particles.Box lattice (console 's', Particles.cc:167-170). */
+int pbd_spawn_box(pbd_ctx* ctx);
+/* N = 0 (clearParticlesCallback, kernel/Communication.h:657-660). */
+int pbd_clear_particles(pbd_ctx* ctx);
+/* Advance `frames` frames; each frame is `substeps` calls of the step with
+ * dt = delta_t / substeps (Fluid::compute, Fluid.cu:14-97).  Stream-ordered:
+ * returns after enqueueing; call pbd_sync to wait (Simulation.cu:18). */
+int pbd_step(pbd_ctx* ctx, uint32_t frames);
+/* Exactly one call of the step (one substep, dt = delta_t / substeps):
+ * Fluid::compute (Fluid.cu:14-97). */
+int pbd_compute(pbd_ctx* ctx);
+/* Same as pbd_step but replays a captured hipGraph of one substep. */
+int pbd_step_graph(pbd_ctx* ctx, uint32_t frames);
+int pbd_sync(pbd_ctx* ctx);
+int pbd_num_particles(pbd_ctx* ctx, uint32_t* n_out);
+int pbd_get_buffers(pbd_ctx* ctx, pbd_buffers* out);
+/* Synchronous host copies of one buffer (PBD_BUF_*); bytes must not exceed
+ * the buffer size.  Upload of PBD_BUF_POSITIONS..PBD_BUF_EXTERNAL_FORCES only. */
+int pbd_download(pbd_ctx* ctx, int which, void* dst, size_t bytes);
+int pbd_upload(pbd_ctx* ctx, int which, const void* src, size_t bytes);
+/* Size in bytes of buffer `which` for the current particle count. */
+int pbd_buffer_bytes(pbd_ctx* ctx, int which, size_t* bytes);
+/* Scan positions/velocities for NaN/Inf; PBD_ERR_NONFINITE if any. */
+int pbd_check_finite(pbd_ctx* ctx, uint32_t* bad_count);
+/* Per-stage timing with HIP events on the context stream (0 = off). */
+int pbd_set_timing(pbd_ctx* ctx, int enabled);
+/* Stage names and accumulated milliseconds / launch counts since the last
+ * reset; returns the number of stages in *count. */
+int pbd_get_timings(pbd_ctx* ctx, const char** names, double* ms, uint64_t* launches,
+                    uint32_t cap, uint32_t* count);
+int pbd_reset_timings(pbd_ctx* ctx);
+/* Last error message of this context (or of the last failed pbd_create when ctx is NULL). */
+const char* pbd_last_error(pbd_ctx* ctx);
+/* Free the context (cudaCleanupKernels, kernel/Initialize.h:136-140). */
+void pbd_destroy(pbd_ctx* ctx);
+/* ABI version (PBD_ABI_VERSION) and the gfx target the kernels were built for. */
+int pbd_version(void);
+const char* pbd_build_target(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBD_PBD_H */

@@ -1,0 +1,217 @@
+// capi.cpp - the extern "C" boundary (include/pbd/pbd.h).  No exception
+// crosses it: every entry point returns a status code.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "fluid.hpp"
+#include "params.hpp"
+#include "pbd/pbd.h"
+
+struct pbd_ctx {
+  pbd::Fluid fluid;
+};
+
+namespace {
+thread_local std::string g_create_error;
+
+int guard_status(pbd_ctx* ctx, int rc) {
+  (void)ctx;
+  return rc;
+}
+}  // namespace
+
+#define PBD_TRY(ctx, ...)                                                    \
+  try {                                                                      \
+    __VA_ARGS__                                                              \
+  } catch (const std::bad_alloc&) {                                          \
+    return PBD_ERR_NOMEM;                                                    \
+  } catch (...) {                                                            \
+    return PBD_ERR_INVALID;                                                  \
+  }
+
+extern "C" {
+
+int pbd_version(void) { return PBD_ABI_VERSION; }
+
+const char* pbd_build_target(void) { return "gfx950"; }
+
+uint32_t pbd_morton_encode(uint32_t x, uint32_t y, uint32_t z) { return pbd::morton3(x, y, z); }
+
+int pbd_cell_key(const pbd_params* p, const float pos[3], uint32_t* key, uint32_t ijk[3]) {
+  if (!p || !pos) return PBD_ERR_INVALID;
+  pbd::DevParams d;
+  const char* why = nullptr;
+  if (pbd::derive_device_params(*p, 0, d, &why) != PBD_OK) return PBD_ERR_INVALID;
+  uint32_t ix, iy, iz;
+  uint32_t k = pbd::cell_key(d, pos[0], pos[1], pos[2], ix, iy, iz);
+  if (key) *key = k;
+  if (ijk) {
+    ijk[0] = ix;
+    ijk[1] = iy;
+    ijk[2] = iz;
+  }
+  return PBD_OK;
+}
+
+int pbd_create(const pbd_params* p, uint32_t max_particles, pbd_ctx** out) {
+  if (!p || !out) return PBD_ERR_INVALID;
+  *out = nullptr;
+  PBD_TRY(nullptr, {
+    pbd_ctx* c = new pbd_ctx();
+    int rc = c->fluid.init(*p, max_particles);
+    if (rc != PBD_OK) {
+      g_create_error = c->fluid.error();
+      delete c;
+      return rc;
+    }
+    *out = c;
+    // Particles.cc:18-22: spawn the box at construction when requested
+    if (p->box_spawn_in_the_beginning) {
+      rc = pbd_spawn_box(c);
+      if (rc != PBD_OK) {
+        g_create_error = c->fluid.error().empty() ? "spawning the initial box failed" : c->fluid.error();
+        delete c;
+        *out = nullptr;
+        return rc;
+      }
+    }
+    return PBD_OK;
+  })
+}
+
+int pbd_set_params(pbd_ctx* ctx, const pbd_params* p) {
+  if (!ctx || !p) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, {
+    ctx->fluid.clear_status();
+    return ctx->fluid.set_params(*p);
+  })
+}
+
+int pbd_get_params(pbd_ctx* ctx, pbd_params* out) {
+  if (!ctx || !out) return PBD_ERR_INVALID;
+  *out = ctx->fluid.params();
+  return PBD_OK;
+}
+
+int pbd_add_particles(pbd_ctx* ctx, uint32_t n, const float* pos4, const float* vel4, const float* col4) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, {
+    ctx->fluid.clear_status();
+    return ctx->fluid.add_particles(n, pos4, vel4, col4);
+  })
+}
+
+int pbd_add_particle(pbd_ctx* ctx, const float pos[3], const float dir[3]) {
+  if (!ctx || !pos || !dir) return PBD_ERR_INVALID;
+  // addParticle (Communication.h:22-38): position (pos, 0), velocity 50 * (dir, 0), red
+  const float p4[4] = {pos[0], pos[1], pos[2], 0.0f};
+  const float v4[4] = {50.0f * dir[0], 50.0f * dir[1], 50.0f * dir[2], 0.0f};
+  const float c4[4] = {1.0f, 0.0f, 0.0f, 1.0f};
+  return pbd_add_particles(ctx, 1, p4, v4, c4);
+}
+
+int pbd_spawn_box(pbd_ctx* ctx) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, {
+    const pbd_params& p = ctx->fluid.params();
+    uint64_t total = (uint64_t)p.box_width * p.box_height * p.box_depth;
+    if (total + ctx->fluid.num_particles() > ctx->fluid.capacity()) return PBD_ERR_CAPACITY;
+    std::vector<float> pos(total * 4), vel(total * 4), col(total * 4);
+    uint32_t n = 0;
+    int rc = pbd_generate_box(&p, pos.data(), vel.data(), col.data(), (uint32_t)total, &n);
+    if (rc != PBD_OK) return rc;
+    return pbd_add_particles(ctx, n, pos.data(), vel.data(), col.data());
+  })
+}
+
+int pbd_clear_particles(pbd_ctx* ctx) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, { return ctx->fluid.clear_particles(); })
+}
+
+int pbd_step(pbd_ctx* ctx, uint32_t frames) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, { return guard_status(ctx, ctx->fluid.step(frames, false)); })
+}
+
+int pbd_compute(pbd_ctx* ctx) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, {
+    ctx->fluid.compute();
+    return ctx->fluid.status();
+  })
+}
+
+int pbd_step_graph(pbd_ctx* ctx, uint32_t frames) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, { return guard_status(ctx, ctx->fluid.step(frames, true)); })
+}
+
+int pbd_sync(pbd_ctx* ctx) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, { return ctx->fluid.sync(); })
+}
+
+int pbd_num_particles(pbd_ctx* ctx, uint32_t* n_out) {
+  if (!ctx || !n_out) return PBD_ERR_INVALID;
+  *n_out = ctx->fluid.num_particles();
+  return PBD_OK;
+}
+
+int pbd_get_buffers(pbd_ctx* ctx, pbd_buffers* out) {
+  if (!ctx || !out) return PBD_ERR_INVALID;
+  ctx->fluid.get_buffers(out);
+  return PBD_OK;
+}
+
+int pbd_download(pbd_ctx* ctx, int which, void* dst, size_t bytes) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, { return ctx->fluid.download(which, dst, bytes); })
+}
+
+int pbd_upload(pbd_ctx* ctx, int which, const void* src, size_t bytes) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, { return ctx->fluid.upload(which, src, bytes); })
+}
+
+int pbd_buffer_bytes(pbd_ctx* ctx, int which, size_t* bytes) {
+  if (!ctx || !bytes) return PBD_ERR_INVALID;
+  return ctx->fluid.buffer_bytes(which, bytes);
+}
+
+int pbd_check_finite(pbd_ctx* ctx, uint32_t* bad_count) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, { return ctx->fluid.check_finite(bad_count); })
+}
+
+int pbd_set_timing(pbd_ctx* ctx, int enabled) {
+  if (!ctx) return PBD_ERR_INVALID;
+  ctx->fluid.set_timing(enabled != 0);
+  return PBD_OK;
+}
+
+int pbd_get_timings(pbd_ctx* ctx, const char** names, double* ms, uint64_t* launches, uint32_t cap, uint32_t* count) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, { return ctx->fluid.timings(names, ms, launches, cap, count); })
+}
+
+int pbd_reset_timings(pbd_ctx* ctx) {
+  if (!ctx) return PBD_ERR_INVALID;
+  ctx->fluid.reset_timings();
+  return PBD_OK;
+}
+
+const char* pbd_last_error(pbd_ctx* ctx) {
+  if (!ctx) return g_create_error.c_str();
+  return ctx->fluid.error().c_str();
+}
+
+void pbd_destroy(pbd_ctx* ctx) { delete ctx; }
+
+}  // extern "C"

@@ -1,0 +1,875 @@
+// config.cpp - config.txt reader/writer, parameter table, scene generator and
+// derivation of the per-step device constants.  Host only; no GPU needed.
+//
+// Grammar (restated from src/parser/*.h of the reference):
+//   - CommentParser.h:24-58     '//' to end of line and '/* ... */' across lines
+//   - IncludeParser.h:30-60     '#include "file"' relative to the including file, once per file
+//   - DefineParser.h:115-153    '#define NAME rest-of-line', '#undef NAME', token-bounded
+//                               substitution; Config.h:108-109 predefines true=1, false=0
+//   - ConfigParser.h:392-457    'Name:' opens a scope entered by '{', closed by '}';
+//                               'key = value;' defines; "strings" and [arrays] are kept raw;
+//                               blanks outside strings are dropped
+//   - ConfigParser.h:151-182    getValue<T>: strip quotes and brackets, stream-read T
+//   - ConfigObject.h:564-591    write(): defines sorted by name, then child scopes sorted
+#include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "params.hpp"
+#include "pbd/pbd.h"
+
+namespace pbd {
+namespace {
+
+void set_err(char* err, size_t len, const std::string& msg) {
+  if (err && len) {
+    std::snprintf(err, len, "%s", msg.c_str());
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scoped key/value tree (ConfigObject.h)
+struct Scope {
+  std::map<std::string, std::string> defines;
+  std::map<std::string, std::unique_ptr<Scope>> children;
+  Scope* child(const std::string& name) {
+    auto& c = children[name];
+    if (!c) c.reset(new Scope());
+    return c.get();
+  }
+};
+
+struct ParseError {
+  std::string msg;
+};
+
+bool is_word_char(char c) {
+  return (c >= 'A' && c <= 'Z') || (c >= 'a' && c <= 'z') || (c >= '0' && c <= '9') || c == '_';
+}
+
+// Token-bounded replacement of every define (DefineParser.h:157-179).
+std::string apply_defines(const std::string& line, const std::map<std::string, std::string>& defs) {
+  std::string s = line;
+  for (const auto& d : defs) {
+    const std::string& name = d.first;
+    if (name.empty()) continue;
+    std::string out;
+    size_t i = 0;
+    while (i < s.size()) {
+      size_t pos = s.find(name, i);
+      if (pos == std::string::npos) {
+        out.append(s, i, std::string::npos);
+        break;
+      }
+      bool left_ok = pos == 0 || !is_word_char(s[pos - 1]);
+      size_t after = pos + name.size();
+      bool right_ok = after >= s.size() || !is_word_char(s[after]);
+      out.append(s, i, pos - i);
+      if (left_ok && right_ok) {
+        out += d.second;
+      } else {
+        out += name;
+      }
+      i = after;
+    }
+    s = out;
+  }
+  return s;
+}
+
+class Preprocessor {
+ public:
+  Preprocessor() {
+    defines_["true"] = "1";
+    defines_["false"] = "0";
+  }
+  std::string run(const std::string& path) {
+    std::string out;
+    include(path, out);
+    return out;
+  }
+
+ private:
+  void include(const std::string& path, std::string& out) {
+    if (std::find(included_.begin(), included_.end(), path) != included_.end()) return;
+    included_.push_back(path);
+    std::ifstream in(path.c_str());
+    if (!in) throw ParseError{"Could not open file '" + path + "'"};
+    std::string dir;
+    size_t slash = path.rfind('/');
+    if (slash != std::string::npos) dir = path.substr(0, slash + 1);
+    std::string line;
+    bool in_comment = false;
+    while (std::getline(in, line)) {
+      // comments
+      std::string clean;
+      for (size_t i = 0; i < line.size(); ++i) {
+        char a = line[i], b = i + 1 < line.size() ? line[i + 1] : '\0';
+        if (in_comment) {
+          if (a == '*' && b == '/') {
+            in_comment = false;
+            ++i;
+          }
+          continue;
+        }
+        if (a == '/' && b == '*') {
+          in_comment = true;
+          ++i;
+          continue;
+        }
+        if (a == '/' && b == '/') break;
+        clean += a;
+      }
+      // defines (substitution first, then directives)
+      std::string sub = apply_defines(clean, defines_);
+      std::istringstream is(sub);
+      std::string tok;
+      if (is >> tok) {
+        if (tok == "#define") {
+          std::string name, rest;
+          is >> name >> std::ws;
+          std::getline(is, rest);
+          defines_[name] = rest;
+          out += "\n";
+          continue;
+        }
+        if (tok == "#undef") {
+          std::string name;
+          is >> name;
+          defines_.erase(name);
+          out += "\n";
+          continue;
+        }
+        if (tok == "#include") {
+          std::string file;
+          is >> file;
+          if (file.size() >= 2 && (file.front() == '"' || file.front() == '<')) file = file.substr(1, file.size() - 2);
+          include(dir + file, out);
+          out += "\n";
+          continue;
+        }
+      }
+      out += sub;
+      out += "\n";
+    }
+  }
+  std::map<std::string, std::string> defines_;
+  std::vector<std::string> included_;
+};
+
+// ConfigParser::actualLineParsing restated over the whole preprocessed text.
+void parse_tree(const std::string& text, Scope& root) {
+  std::vector<Scope*> stack{&root};
+  Scope* active = &root;
+  Scope* declared = nullptr;
+  std::string buf;
+  bool in_string = false, in_array = false;
+  int line_no = 1;
+  for (char c : text) {
+    if (c == '\n') ++line_no;
+    if (c == '"') {
+      in_string = !in_string;
+      buf += c;
+    } else if (in_string) {
+      buf += c;
+    } else if (c == ':') {
+      declared = active->child(buf);
+      stack.push_back(declared);
+      buf.clear();
+    } else if (c == '[') {
+      in_array = true;
+      buf += c;
+    } else if (c == ']') {
+      in_array = false;
+      buf += c;
+    } else if (in_array) {
+      if (!std::isspace((unsigned char)c)) buf += c;
+    } else if (c == '{') {
+      active = stack.back();
+      buf.clear();
+    } else if (c == '}') {
+      if (stack.size() <= 1) throw ParseError{"Unbalanced '}' at line " + std::to_string(line_no)};
+      stack.pop_back();
+      active = stack.back();
+      buf.clear();
+    } else if (c == ';') {
+      size_t eq = buf.find('=');
+      std::string var = eq == std::string::npos ? buf : buf.substr(0, eq);
+      std::string val = eq == std::string::npos ? std::string() : buf.substr(eq + 1);
+      if (!var.empty() && !val.empty()) {
+        active->defines[var] = val;
+      } else if (!var.empty() && val.empty()) {
+        throw ParseError{"Uninitialized value in config file @ line " + std::to_string(line_no)};
+      } else if (var.empty() && !val.empty()) {
+        throw ParseError{"Invalid config file @ line " + std::to_string(line_no)};
+      }
+      buf.clear();
+    } else if (std::isspace((unsigned char)c)) {
+      // blanks outside strings and arrays are dropped
+    } else {
+      buf += c;
+    }
+  }
+  if (in_string) throw ParseError{"Unterminated string"};
+}
+
+std::vector<std::string> split_path(const std::string& key) {
+  std::vector<std::string> out;
+  std::string part;
+  std::istringstream is(key);
+  while (std::getline(is, part, '.')) out.push_back(part);
+  return out;
+}
+
+bool tree_get(Scope& root, const std::string& key, std::string& val) {
+  auto path = split_path(key);
+  if (path.empty()) return false;
+  Scope* s = &root;
+  for (size_t i = 0; i + 1 < path.size(); ++i) {
+    auto it = s->children.find(path[i]);
+    if (it == s->children.end()) return false;
+    s = it->second.get();
+  }
+  auto it = s->defines.find(path.back());
+  if (it == s->defines.end()) return false;
+  val = it->second;
+  return true;
+}
+
+std::string strip_value(const std::string& raw) {
+  std::string s;
+  for (char c : raw)
+    if (c != '"' && c != '[' && c != ']') s += c;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Parameter table: dotted key -> pbd_params field.
+enum class KT { U32, I32, F32, BOOL, F32X3, U32X3, SEED, VARIANT };
+
+struct KeyDesc {
+  const char* key;
+  KT type;
+  size_t off;
+  bool reference;    // read by the reference (missing -> error, like ConfigParser.h:160-171)
+  bool ranged;       // console setter range check (Communication.h:440-624)
+  double lo, hi;
+  bool lo_open, hi_open;
+};
+
+#define OFF(f) offsetof(pbd_params, f)
+const KeyDesc kKeys[] = {
+    {"Application.Cuda.threadsPerBlock", KT::U32, OFF(threads_per_block), true, true, 64, 1024, false, false},
+    {"Application.OpenGL.Textures.width", KT::U32, OFF(texture_width), true, true, 1, 65535, false, false},
+    {"Application.Simulation.Active.applyDeltaPositions", KT::BOOL, OFF(active_apply_delta_positions), true, false, 0, 0, false, false},
+    {"Application.Simulation.Active.applyForces", KT::BOOL, OFF(active_apply_forces), true, false, 0, 0, false, false},
+    {"Application.Simulation.Active.computeDeltaPositions", KT::BOOL, OFF(active_compute_delta_positions), true, false, 0, 0, false, false},
+    {"Application.Simulation.Active.computeLambda", KT::BOOL, OFF(active_compute_lambda), true, false, 0, 0, false, false},
+    {"Application.Simulation.Active.computeOmegas", KT::BOOL, OFF(active_compute_omegas), true, false, 0, 0, false, false},
+    {"Application.Simulation.Active.computeViscosity", KT::BOOL, OFF(active_compute_viscosity), true, false, 0, 0, false, false},
+    {"Application.Simulation.Active.computeVorticity", KT::BOOL, OFF(active_compute_vorticity), true, false, 0, 0, false, false},
+    {"Application.Simulation.Active.findContacts", KT::BOOL, OFF(active_find_contacts), true, false, 0, 0, false, false},
+    {"Application.Simulation.Active.findNeighbours", KT::BOOL, OFF(active_find_neighbours), true, false, 0, 0, false, false},
+    {"Application.Simulation.Active.updatePositions", KT::BOOL, OFF(active_update_positions), true, false, 0, 0, false, false},
+    {"Application.Simulation.Collision.collisionType", KT::I32, OFF(collision_type), true, true, 0, 1, false, false},
+    {"Application.Simulation.Collision.kernelWidthNeighbours", KT::U32, OFF(kernel_width_neighbours), true, true, 1, 50, false, false},
+    {"Application.Simulation.Collision.maxCollisionBatches", KT::U32, OFF(max_collision_batches), true, true, 1, 512, false, false},
+    {"Application.Simulation.Collision.stiffness", KT::F32, OFF(stiffness), true, true, 0, 1, false, false},
+    {"Application.Simulation.Density.kSCorr", KT::F32, OFF(k_scorr), true, true, -100, 100, false, false},
+    {"Application.Simulation.Density.kernelWidthSpiky", KT::F32, OFF(kernel_width_spiky), true, true, 0, 1000, true, false},
+    {"Application.Simulation.Density.nSCorr", KT::I32, OFF(n_scorr), true, true, -100, 100, false, false},
+    {"Application.Simulation.Density.qSCorr", KT::F32, OFF(q_scorr), true, true, -100, 100, false, false},
+    {"Application.Simulation.Density.restDensity", KT::F32, OFF(rest_density), true, true, 0, 100000, true, false},
+    {"Application.Simulation.Density.relaxation", KT::F32, OFF(relaxation), false, true, 0, 1e9, true, false},
+    {"Application.Simulation.Density.restDensityFromLattice", KT::BOOL, OFF(rest_density_from_lattice), false, false, 0, 0, false, false},
+    {"Application.Simulation.Density.kernelVariant", KT::VARIANT, OFF(kernel_variant), false, false, 0, 0, false, false},
+    {"Application.Simulation.Enclosure.enclosurePositionDamping", KT::F32, OFF(enclosure_position_damping), true, true, 0, 1, false, false},
+    {"Application.Simulation.Enclosure.enclosureVelocityDamping", KT::F32, OFF(enclosure_velocity_damping), true, true, 0, 1, false, false},
+    {"Application.Simulation.Enclosure.X.min", KT::F32, OFF(bounds_min) + 0 * sizeof(float), true, false, 0, 0, false, false},
+    {"Application.Simulation.Enclosure.X.max", KT::F32, OFF(bounds_max) + 0 * sizeof(float), true, false, 0, 0, false, false},
+    {"Application.Simulation.Enclosure.Y.min", KT::F32, OFF(bounds_min) + 1 * sizeof(float), true, false, 0, 0, false, false},
+    {"Application.Simulation.Enclosure.Y.max", KT::F32, OFF(bounds_max) + 1 * sizeof(float), true, false, 0, 0, false, false},
+    {"Application.Simulation.Enclosure.Z.min", KT::F32, OFF(bounds_min) + 2 * sizeof(float), true, false, 0, 0, false, false},
+    {"Application.Simulation.Enclosure.Z.max", KT::F32, OFF(bounds_max) + 2 * sizeof(float), true, false, 0, 0, false, false},
+    {"Application.Simulation.Forces.deltaT", KT::F32, OFF(delta_t), true, true, 0, 10, true, false},
+    {"Application.Simulation.Forces.gravity", KT::F32, OFF(gravity), true, true, -1000, 1000, false, false},
+    {"Application.Simulation.Forces.substeps", KT::U32, OFF(substeps), false, true, 1, 1000, false, false},
+    {"Application.Simulation.Iterations.solverIterations", KT::U32, OFF(solver_iterations), true, true, 0, 100, false, false},
+    {"Application.Simulation.Iterations.stabilizationIterations", KT::U32, OFF(stabilization_iterations), true, true, 0, 100, false, false},
+    {"Application.Simulation.Particles.maxNeighboursPerParticle", KT::U32, OFF(max_neighbours_per_particle), true, true, 1, 4096, false, false},
+    {"Application.Simulation.Particles.particleRadius", KT::F32, OFF(particle_radius), true, true, 0, 1e6, true, false},
+    {"Application.Simulation.Particles.Box.width", KT::U32, OFF(box_width), true, true, 1, 1024, false, false},
+    {"Application.Simulation.Particles.Box.height", KT::U32, OFF(box_height), true, true, 1, 1024, false, false},
+    {"Application.Simulation.Particles.Box.depth", KT::U32, OFF(box_depth), true, true, 1, 1024, false, false},
+    {"Application.Simulation.Particles.Box.spacing", KT::F32, OFF(box_spacing), true, false, 0, 0, false, false},
+    {"Application.Simulation.Particles.Box.spawnInTheBeginning", KT::BOOL, OFF(box_spawn_in_the_beginning), true, false, 0, 0, false, false},
+    {"Application.Simulation.Particles.Box.origin", KT::F32X3, OFF(box_origin), false, false, 0, 0, false, false},
+    {"Application.Simulation.Particles.Box.jitter", KT::F32, OFF(box_jitter), false, true, 0, 1e6, false, false},
+    {"Application.Simulation.Particles.Box.seed", KT::SEED, OFF(box_seed_lo), false, false, 0, 0, false, false},
+    {"Application.Simulation.Viscosity.cViscosity", KT::F32, OFF(c_viscosity), true, true, 0, 100, false, false},
+    {"Application.Simulation.Viscosity.kernelWidthPoly", KT::F32, OFF(kernel_width_poly), true, true, 0, 1000, true, false},
+    {"Application.Simulation.Vorticity.eVorticity", KT::F32, OFF(e_vorticity), true, true, -10000, 10000, false, false},
+    {"Application.Simulation.Grid.cellSize", KT::F32, OFF(grid_cell_size), false, false, 0, 0, false, false},
+    {"Application.Simulation.Grid.origin", KT::F32X3, OFF(grid_origin), false, false, 0, 0, false, false},
+    {"Application.Simulation.Grid.dims", KT::U32X3, OFF(grid_dims), false, false, 0, 0, false, false},
+};
+#undef OFF
+
+const KeyDesc* find_key(const std::string& key) {
+  for (const auto& k : kKeys)
+    if (key == k.key) return &k;
+  return nullptr;
+}
+
+template <class T>
+T& field(pbd_params* p, size_t off) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + off);
+}
+template <class T>
+const T& field(const pbd_params* p, size_t off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + off);
+}
+
+bool parse_double(const std::string& s, double& v) {
+  const char* b = s.c_str();
+  while (*b && std::isspace((unsigned char)*b)) ++b;
+  char* e = nullptr;
+  errno = 0;
+  v = std::strtod(b, &e);
+  return e != b && errno == 0;
+}
+
+bool parse_list(const std::string& s, std::vector<double>& out) {
+  out.clear();
+  std::string part;
+  std::istringstream is(s);
+  while (std::getline(is, part, ',')) {
+    double v;
+    if (!parse_double(part, v)) return false;
+    out.push_back(v);
+  }
+  return true;
+}
+
+bool in_range(const KeyDesc& k, double v) {
+  if (!k.ranged) return true;
+  bool lo_ok = k.lo_open ? v > k.lo : v >= k.lo;
+  bool hi_ok = k.hi_open ? v < k.hi : v <= k.hi;
+  return lo_ok && hi_ok;
+}
+
+// Assign text to a key.  check_range applies the console's ranges.
+int assign(pbd_params* p, const KeyDesc& k, const std::string& raw, bool check_range, std::string& why) {
+  std::string v = strip_value(raw);
+  double d = 0;
+  switch (k.type) {
+    case KT::U32:
+    case KT::I32:
+    case KT::F32:
+    case KT::BOOL: {
+      if (!parse_double(v, d)) {
+        why = std::string("cannot parse value '") + raw + "' of '" + k.key + "'";
+        return PBD_ERR_CONFIG;
+      }
+      if (check_range && !in_range(k, d)) {
+        std::ostringstream os;
+        os << "Value out of range " << (k.lo_open ? "]" : "[") << k.lo << ", " << k.hi << (k.hi_open ? "[" : "]")
+           << " for '" << k.key << "'";
+        why = os.str();
+        return PBD_ERR_INVALID;
+      }
+      if (k.type == KT::U32) {
+        if (d < 0) {
+          why = std::string("negative value for unsigned key '") + k.key + "'";
+          return PBD_ERR_INVALID;
+        }
+        field<uint32_t>(p, k.off) = (uint32_t)d;
+      } else if (k.type == KT::I32) {
+        field<int32_t>(p, k.off) = (int32_t)d;
+      } else if (k.type == KT::F32) {
+        field<float>(p, k.off) = (float)d;
+      } else {
+        field<int32_t>(p, k.off) = d != 0.0 ? 1 : 0;
+      }
+      return PBD_OK;
+    }
+    case KT::F32X3:
+    case KT::U32X3: {
+      std::vector<double> xs;
+      if (!parse_list(v, xs) || xs.size() != 3) {
+        why = std::string("Failed to retrive array '") + k.key + "'; the array contains the wrong number of elements";
+        return PBD_ERR_CONFIG;
+      }
+      for (int a = 0; a < 3; ++a) {
+        if (k.type == KT::F32X3) {
+          field<float>(p, k.off + a * sizeof(float)) = (float)xs[a];
+        } else {
+          if (xs[a] < 1 || xs[a] > 1024) {
+            why = std::string("grid dims must be in [1, 1024] for '") + k.key + "'";
+            return PBD_ERR_INVALID;
+          }
+          field<uint32_t>(p, k.off + a * sizeof(uint32_t)) = (uint32_t)xs[a];
+        }
+      }
+      return PBD_OK;
+    }
+    case KT::SEED: {
+      const char* b = v.c_str();
+      char* e = nullptr;
+      errno = 0;
+      unsigned long long s = std::strtoull(b, &e, 0);
+      if (e == b || errno) {
+        why = std::string("cannot parse seed '") + raw + "'";
+        return PBD_ERR_CONFIG;
+      }
+      p->box_seed_lo = (uint32_t)(s & 0xFFFFFFFFull);
+      p->box_seed_hi = (uint32_t)(s >> 32);
+      return PBD_OK;
+    }
+    case KT::VARIANT: {
+      std::string t = v;
+      std::transform(t.begin(), t.end(), t.begin(), ::tolower);
+      if (t == "reference" || t == "0") {
+        p->kernel_variant = PBD_KERNEL_REFERENCE;
+      } else if (t == "textbook" || t == "1") {
+        p->kernel_variant = PBD_KERNEL_TEXTBOOK;
+      } else {
+        why = "kernelVariant must be 'reference' or 'textbook'";
+        return PBD_ERR_INVALID;
+      }
+      return PBD_OK;
+    }
+  }
+  return PBD_ERR_INVALID;
+}
+
+std::string format_value(const pbd_params* p, const KeyDesc& k) {
+  std::ostringstream os;
+  os.precision(9);
+  switch (k.type) {
+    case KT::U32:
+      os << field<uint32_t>(p, k.off);
+      break;
+    case KT::I32:
+    case KT::BOOL:
+      os << field<int32_t>(p, k.off);
+      break;
+    case KT::F32:
+      os << field<float>(p, k.off);
+      break;
+    case KT::F32X3:
+      os << "[" << field<float>(p, k.off) << "," << field<float>(p, k.off + 4) << "," << field<float>(p, k.off + 8)
+         << "]";
+      break;
+    case KT::U32X3:
+      os << "[" << field<uint32_t>(p, k.off) << "," << field<uint32_t>(p, k.off + 4) << ","
+         << field<uint32_t>(p, k.off + 8) << "]";
+      break;
+    case KT::SEED:
+      os << (((unsigned long long)p->box_seed_hi << 32) | p->box_seed_lo);
+      break;
+    case KT::VARIANT:
+      os << "\"" << (p->kernel_variant == PBD_KERNEL_TEXTBOOK ? "textbook" : "reference") << "\"";
+      break;
+  }
+  return os.str();
+}
+
+// Enclosure setters (Communication.h:573-598): min < max inside the grid with
+// a 1.5-cell margin ([1.5, 126.5] for the reference's 128 unit cells).
+int check_enclosure(const pbd_params* p, int axis, std::string& why) {
+  float cell = p->grid_cell_size > 0 ? p->grid_cell_size : 2.0f * p->particle_radius;
+  double lo = p->grid_origin[axis] + 1.5 * cell;
+  double hi = p->grid_origin[axis] + (p->grid_dims[axis] - 1.5) * cell;
+  double mn = p->bounds_min[axis], mx = p->bounds_max[axis];
+  if (!(mn < mx && mn >= lo - 1e-6 && mx <= hi + 1e-6)) {
+    std::ostringstream os;
+    os << "Values out of range, each should be in [" << lo << ", " << hi << "].";
+    why = os.str();
+    return PBD_ERR_INVALID;
+  }
+  return PBD_OK;
+}
+
+void write_scope(std::ostringstream& os, const Scope& s) {
+  for (const auto& d : s.defines) os << d.first << " = " << d.second << ";\n";
+  if (!s.defines.empty() && !s.children.empty()) os << "\n";
+  size_t i = 0;
+  for (const auto& c : s.children) {
+    os << c.first << ":\n{\n";
+    std::ostringstream inner;
+    write_scope(inner, *c.second);
+    std::istringstream lines(inner.str());
+    std::string line;
+    while (std::getline(lines, line)) os << (line.empty() ? "" : "  ") << line << "\n";
+    os << "}\n";
+    if (++i != s.children.size()) os << "\n";
+  }
+}
+
+void tree_put(Scope& root, const std::string& key, const std::string& value) {
+  auto path = split_path(key);
+  Scope* s = &root;
+  for (size_t i = 0; i + 1 < path.size(); ++i) s = s->child(path[i]);
+  s->defines[path.back()] = value;
+}
+
+// splitmix64 (Steele, Lea, Flood 2014), the scene generator's RNG.
+inline uint64_t splitmix64(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline double uniform01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+constexpr double kPiRef = 3.14159265359;  // M_PI of kernel/Globals.h:122
+constexpr double kPi = 3.14159265358979323846;
+
+double poly6(int variant, double h, double r) {
+  if (variant == PBD_KERNEL_REFERENCE) {
+    if (r > 0.001 && r < h - 0.001) {
+      double t = h * h - r * r;
+      double t3 = t * t * t;
+      return 315.0 * t3 * t3 / (0.001 + 64.0 * kPiRef * std::pow(h, 9));
+    }
+    return 0.0;
+  }
+  if (r >= 0.0 && r < h) {
+    double t = h * h - r * r;
+    return 315.0 / (64.0 * kPi * std::pow(h, 9)) * t * t * t;
+  }
+  return 0.0;
+}
+
+double lattice_density(int variant, double h, double s) {
+  if (!(s > 0)) return 0.0;
+  int m = (int)std::ceil(h / s) + 1;
+  double sum = 0.0;
+  for (int a = -m; a <= m; ++a)
+    for (int b = -m; b <= m; ++b)
+      for (int c = -m; c <= m; ++c) sum += poly6(variant, h, s * std::sqrt((double)(a * a + b * b + c * c)));
+  return sum;
+}
+
+}  // namespace
+
+int derive_device_params(const pbd_params& p, uint32_t n, DevParams& o, const char** why) {
+  std::memset(&o, 0, sizeof(o));
+  if (p.kernel_variant != PBD_KERNEL_REFERENCE && p.kernel_variant != PBD_KERNEL_TEXTBOOK) {
+    *why = "kernel_variant must be 0 (reference) or 1 (textbook)";
+    return PBD_ERR_INVALID;
+  }
+  if (p.max_neighbours_per_particle == 0) {
+    *why = "maxNeighboursPerParticle must be >= 1";
+    return PBD_ERR_INVALID;
+  }
+  if (p.substeps == 0) {
+    *why = "substeps must be >= 1";
+    return PBD_ERR_INVALID;
+  }
+  if (p.collision_type != 0 && p.collision_type != 1) {
+    *why = "collisionType must be 0 or 1";
+    return PBD_ERR_INVALID;
+  }
+  for (int a = 0; a < 3; ++a)
+    if (p.grid_dims[a] < 1 || p.grid_dims[a] > 1024) {
+      *why = "grid dims must be in [1, 1024]";
+      return PBD_ERR_INVALID;
+    }
+  const float diam = 2.0f * p.particle_radius;  // Initialize.h:43
+  const float hs = diam * p.kernel_width_spiky;  // Initialize.h:45
+  const float dq = hs * p.q_scorr;               // Initialize.h:49
+  const float hp = p.kernel_width_poly;          // Initialize.h:53
+  const float cell = p.grid_cell_size > 0.0f ? p.grid_cell_size : diam;
+  if (!(cell > 0.0f) || !(hs > 0.0f) || !(hp > 0.0f)) {
+    *why = "cell size and kernel widths must be positive";
+    return PBD_ERR_INVALID;
+  }
+  o.n = n;
+  o.cap = p.max_neighbours_per_particle;
+  o.kw = (int32_t)p.kernel_width_neighbours;
+  o.variant = p.kernel_variant;
+  uint32_t g = 0;
+  if (p.active_apply_forces) g |= G_APPLY_FORCES;
+  if (p.active_find_contacts) g |= G_FIND_CONTACTS;
+  if (p.active_find_neighbours) g |= G_FIND_NEIGHBOURS;
+  if (p.active_compute_lambda) g |= G_LAMBDA;
+  if (p.active_compute_delta_positions) g |= G_DELTA;
+  if (p.active_apply_delta_positions) g |= G_APPLY_DELTA;
+  if (p.active_update_positions) g |= G_UPDATE;
+  if (p.active_compute_omegas) g |= G_OMEGA;
+  if (p.active_compute_vorticity) g |= G_VORTICITY;
+  if (p.active_compute_viscosity) g |= G_VISCOSITY;
+  o.gates = g;
+  o.dt = p.delta_t / (float)p.substeps;
+  o.inv_dt = 1.0f / o.dt;
+  o.gravity = p.gravity;
+  for (int a = 0; a < 3; ++a) {
+    o.bmin[a] = p.bounds_min[a];
+    o.bmax[a] = p.bounds_max[a];
+    o.lo[a] = p.bounds_min[a] + 0.001f;
+    o.hi[a] = p.bounds_max[a] - 0.001f;
+    o.origin[a] = p.grid_origin[a];
+    o.dims[a] = p.grid_dims[a];
+  }
+  o.vdamp = p.enclosure_velocity_damping;
+  o.pdamp = p.enclosure_position_damping;
+  o.cell = cell;
+  o.inv_cell = 1.0f / cell;
+  o.diameter = diam;
+  o.hp = hp;
+  o.hp2 = hp * hp;
+  o.hs = hs;
+  o.hs2 = hs * hs;
+  const double dhp = hp, dhs = hs;
+  double wdq;
+  if (p.kernel_variant == PBD_KERNEL_TEXTBOOK) {
+    o.wc = (float)(315.0 / (64.0 * kPi * std::pow(dhp, 9)));
+    o.gc = (float)(-45.0 / (kPi * std::pow(dhs, 6)));
+    o.w0 = (float)(315.0 / (64.0 * kPi * std::pow(dhp, 3)));
+  } else {
+    o.wc = (float)(315.0 / (0.001 + 64.0 * kPiRef * std::pow(dhp, 9)));
+    o.gc = (float)(15.0 / (kPiRef * std::pow(dhs, 6)));
+    o.w0 = 0.0f;
+  }
+  wdq = poly6(p.kernel_variant, dhp, (double)dq);
+  o.inv_w_dq = wdq != 0.0 ? (float)(1.0 / wdq) : 0.0f;
+  double rho0 = p.rest_density_from_lattice ? lattice_density(p.kernel_variant, dhp, p.box_spacing) : p.rest_density;
+  if (!(rho0 > 0.0)) {
+    *why = "rest density must be positive";
+    return PBD_ERR_INVALID;
+  }
+  o.rho0 = (float)rho0;
+  o.inv_rho0 = (float)(1.0 / rho0);
+  o.inv_rho0_sq = (float)(1.0 / (rho0 * rho0));
+  o.eps = p.relaxation;
+  // Textbook: k is stated for a unit kernel radius (Macklin & Mueller 2013 eq. 13),
+  // so the artificial pressure is scaled by h^2 to carry the units of lambda.
+  o.k_scorr = p.kernel_variant == PBD_KERNEL_TEXTBOOK ? (float)((double)p.k_scorr * dhs * dhs) : p.k_scorr;
+  o.n_scorr = p.n_scorr;
+  o.coll_scale = -0.5f * (1.0f - p.stiffness);
+  // Textbook XSPH and vorticity carry the SPH particle volume m/rho0 (mass 1).
+  o.vol = p.kernel_variant == PBD_KERNEL_TEXTBOOK ? (float)(1.0 / rho0) : 1.0f;
+  o.c_visc = p.c_viscosity;
+  o.e_vort = p.e_vorticity;
+  return PBD_OK;
+}
+
+}  // namespace pbd
+
+using namespace pbd;
+
+extern "C" size_t pbd_params_size(void) { return sizeof(pbd_params); }
+
+extern "C" int pbd_params_default(pbd_params* o) {
+  if (!o) return PBD_ERR_INVALID;
+  std::memset(o, 0, sizeof(*o));
+  // Values of the reference's config.txt (PBD-Simulation-Framework-CUDA/config.txt:1-133).
+  o->threads_per_block = 512;
+  o->texture_width = 256;
+  o->active_apply_delta_positions = 1;
+  o->active_apply_forces = 1;
+  o->active_compute_delta_positions = 1;
+  o->active_compute_lambda = 1;
+  o->active_compute_omegas = 1;
+  o->active_compute_viscosity = 1;
+  o->active_compute_vorticity = 1;
+  o->active_find_contacts = 1;
+  o->active_find_neighbours = 1;
+  o->active_update_positions = 1;
+  o->collision_type = 0;
+  o->kernel_width_neighbours = 2;
+  o->max_collision_batches = 16;
+  o->stiffness = 0.3f;
+  o->k_scorr = 0.2f;
+  o->kernel_width_spiky = 6.1f;
+  o->n_scorr = 4;
+  o->q_scorr = 0.2f;
+  o->rest_density = 1200.0f;
+  o->relaxation = 1e-4f;  // hard-coded in Density.h:90
+  o->rest_density_from_lattice = 0;
+  o->kernel_variant = PBD_KERNEL_REFERENCE;
+  o->enclosure_position_damping = 0.0f;
+  o->enclosure_velocity_damping = 1.0f;
+  for (int a = 0; a < 3; ++a) {
+    o->bounds_min[a] = 1.5f;
+    o->bounds_max[a] = 62.5f;
+  }
+  o->delta_t = 0.01f;
+  o->gravity = -29.82f;
+  o->substeps = 1;
+  o->solver_iterations = 1;
+  o->stabilization_iterations = 1;
+  o->max_neighbours_per_particle = 512;
+  o->particle_radius = 0.5f;
+  o->box_width = 24;
+  o->box_height = 24;
+  o->box_depth = 24;
+  o->box_spacing = 1.2f;
+  o->box_spawn_in_the_beginning = 0;
+  o->box_origin[0] = 30.0f;  // Particles.cc:197,205: (30 + i s, 2 + 30 + j s, 30 + k s)
+  o->box_origin[1] = 32.0f;
+  o->box_origin[2] = 30.0f;
+  o->box_jitter = 0.0f;
+  o->box_seed_lo = 0;
+  o->box_seed_hi = 0;
+  o->c_viscosity = 0.0001f;
+  o->kernel_width_poly = 4.1f;
+  o->e_vorticity = 1.0f;
+  o->grid_cell_size = 0.0f;  // = 2 r = 1.0: the reference's unit cell (Hash.h:21-29)
+  for (int a = 0; a < 3; ++a) {
+    o->grid_origin[a] = 0.0f;
+    o->grid_dims[a] = 128;  // maxGrid = 128^3 (Particles.cc:8)
+  }
+  return PBD_OK;
+}
+
+static int load_impl(const char* path, pbd_params* out, char* err, size_t err_len, bool strict) {
+  if (!path || !out) return PBD_ERR_INVALID;
+  Scope root;
+  try {
+    Preprocessor pre;
+    std::string text = pre.run(path);
+    parse_tree(text, root);
+  } catch (const ParseError& e) {
+    set_err(err, err_len, e.msg);
+    return PBD_ERR_CONFIG;
+  }
+  pbd_params p = *out;
+  for (const auto& k : kKeys) {
+    std::string raw;
+    if (!tree_get(root, k.key, raw)) {
+      if (strict && k.reference) {
+        std::string spaced = k.key;
+        std::replace(spaced.begin(), spaced.end(), '.', ' ');
+        set_err(err, err_len, "Failed to retrive variable '" + spaced + "' in config '" + path + "'");
+        return PBD_ERR_CONFIG;
+      }
+      continue;
+    }
+    std::string why;
+    int rc = assign(&p, k, raw, false, why);
+    if (rc != PBD_OK) {
+      set_err(err, err_len, why);
+      return rc;
+    }
+  }
+  *out = p;
+  return PBD_OK;
+}
+
+extern "C" int pbd_params_load(const char* path, pbd_params* out, char* err, size_t err_len) {
+  if (!out) return PBD_ERR_INVALID;
+  pbd_params p;
+  pbd_params_default(&p);
+  int rc = load_impl(path, &p, err, err_len, true);
+  if (rc == PBD_OK) *out = p;
+  return rc;
+}
+
+extern "C" int pbd_params_merge(const char* path, pbd_params* inout, char* err, size_t err_len) {
+  return load_impl(path, inout, err, err_len, false);
+}
+
+extern "C" int pbd_params_set(pbd_params* p, const char* key, const char* value, char* err, size_t err_len) {
+  if (!p || !key || !value) return PBD_ERR_INVALID;
+  const KeyDesc* k = find_key(key);
+  if (!k) {
+    set_err(err, err_len, std::string("unknown key '") + key + "'");
+    return PBD_ERR_CONFIG;
+  }
+  pbd_params t = *p;
+  std::string why;
+  int rc = assign(&t, *k, value, true, why);
+  if (rc == PBD_OK) {
+    std::string sk = key;
+    int axis = sk.find("Enclosure.X.") != std::string::npos   ? 0
+               : sk.find("Enclosure.Y.") != std::string::npos ? 1
+               : sk.find("Enclosure.Z.") != std::string::npos ? 2
+                                                              : -1;
+    if (axis >= 0) rc = check_enclosure(&t, axis, why);
+  }
+  if (rc != PBD_OK) {
+    set_err(err, err_len, why);
+    return rc;
+  }
+  *p = t;
+  return PBD_OK;
+}
+
+extern "C" int pbd_params_get(const pbd_params* p, const char* key, char* buf, size_t buf_len) {
+  if (!p || !key) return PBD_ERR_INVALID;
+  const KeyDesc* k = find_key(key);
+  if (!k) return PBD_ERR_CONFIG;
+  std::string v = format_value(p, *k);
+  if (buf && buf_len) std::snprintf(buf, buf_len, "%s", v.c_str());
+  return (buf_len > v.size()) ? PBD_OK : PBD_ERR_INVALID;
+}
+
+extern "C" int pbd_params_write(const pbd_params* p, char* buf, size_t buf_len, size_t* needed) {
+  if (!p) return PBD_ERR_INVALID;
+  Scope root;
+  for (const auto& k : kKeys) tree_put(root, k.key, format_value(p, k));
+  std::ostringstream os;
+  write_scope(os, root);
+  std::string s = os.str();
+  if (needed) *needed = s.size() + 1;
+  if (!buf || buf_len < s.size() + 1) return buf ? PBD_ERR_INVALID : PBD_OK;
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return PBD_OK;
+}
+
+extern "C" int pbd_generate_box(const pbd_params* p, float* pos4, float* vel4, float* col4, uint32_t capacity,
+                                uint32_t* n_out) {
+  if (!p) return PBD_ERR_INVALID;
+  uint64_t total = (uint64_t)p->box_width * p->box_height * p->box_depth;
+  if (n_out) *n_out = (uint32_t)total;
+  if (total > capacity) return PBD_ERR_CAPACITY;
+  uint64_t seed = ((uint64_t)p->box_seed_hi << 32) | p->box_seed_lo;
+  uint64_t st = seed;
+  const double s = p->box_spacing, J = p->box_jitter;
+  uint64_t idx = 0;
+  for (uint32_t i = 0; i < p->box_width; ++i)
+    for (uint32_t j = 0; j < p->box_height; ++j)
+      for (uint32_t k = 0; k < p->box_depth; ++k, ++idx) {
+        double jx = J * (2.0 * uniform01(st) - 1.0);
+        double jy = J * (2.0 * uniform01(st) - 1.0);
+        double jz = J * (2.0 * uniform01(st) - 1.0);
+        double r = uniform01(st), g = uniform01(st), b = uniform01(st);
+        if (pos4) {
+          pos4[4 * idx + 0] = (float)((double)p->box_origin[0] + s * i + jx);
+          pos4[4 * idx + 1] = (float)((double)p->box_origin[1] + s * j + jy);
+          pos4[4 * idx + 2] = (float)((double)p->box_origin[2] + s * k + jz);
+          pos4[4 * idx + 3] = 0.0f;
+        }
+        if (vel4) {
+          vel4[4 * idx + 0] = vel4[4 * idx + 1] = vel4[4 * idx + 2] = vel4[4 * idx + 3] = 0.0f;
+        }
+        if (col4) {
+          col4[4 * idx + 0] = (float)r;
+          col4[4 * idx + 1] = (float)g;
+          col4[4 * idx + 2] = (float)b;
+          col4[4 * idx + 3] = 1.0f;
+        }
+      }
+  return PBD_OK;
+}
+
+extern "C" int pbd_lattice_rest_density(const pbd_params* p, double* out) {
+  if (!p || !out) return PBD_ERR_INVALID;
+  *out = lattice_density(p->kernel_variant, (double)p->kernel_width_poly, (double)p->box_spacing);
+  return PBD_OK;
+}

@@ -1,0 +1,302 @@
+// device.hpp - device (and host-twin) helpers of the PBF step kernels.
+//
+// Everything that decides an integer result (cell ids, sorted order, cell
+// ranges, neighbour counts) is computed here with explicitly rounded FP32
+// operations (no contraction) so that the GPU path and the CPU oracle agree
+// bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "params.hpp"
+
+namespace pbd {
+
+// ---------------------------------------------------------------------------
+// Morton codes (kernel/Hash.h:10-29): bit b of x goes to bit 3b+2, of y to
+// 3b+1, of z to 3b.  Spread/compact by the shift-and-mask ladder.
+constexpr uint32_t kMortonX = 0x24924924u;
+constexpr uint32_t kMortonY = 0x12492492u;
+constexpr uint32_t kMortonZ = 0x09249249u;
+
+__host__ __device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__host__ __device__ __forceinline__ uint32_t compact10(uint32_t v) {
+  v &= 0x09249249u;
+  v = (v | (v >> 2)) & 0x030C30C3u;
+  v = (v | (v >> 4)) & 0x0300F00Fu;
+  v = (v | (v >> 8)) & 0x030000FFu;
+  v = (v | (v >> 16)) & 0x000003FFu;
+  return v;
+}
+
+__host__ __device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return (spread10(x) << 2) | (spread10(y) << 1) | spread10(z);
+}
+
+// Dilated add of a signed cell offset to one Morton component (mask m, shift sh).
+__host__ __device__ __forceinline__ uint32_t dilated_add(uint32_t comp, int d, uint32_t m, uint32_t sh) {
+  if (d >= 0) return ((comp | ~m) + (spread10((uint32_t)d) << sh)) & m;
+  return (comp - (spread10((uint32_t)(-d)) << sh)) & m;
+}
+
+// ---------------------------------------------------------------------------
+// Cell coordinate of one axis: i = (uint)clamp(floor((p - origin) * inv_cell), 0, dim - 1),
+// two rounded FP32 ops then exact floor / clamp (Hash.h:21-26 generalised).
+__host__ __device__ __forceinline__ uint32_t cell_coord(float p, float origin, float inv_cell, uint32_t dim) {
+#pragma clang fp contract(off)
+  float t = (p - origin) * inv_cell;
+  float f = floorf(t);
+  f = fmaxf(f, 0.0f);
+  f = fminf(f, (float)(dim - 1u));
+  return (uint32_t)f;
+}
+
+__host__ __device__ __forceinline__ uint32_t cell_key(const DevParams& P, float x, float y, float z, uint32_t& ix,
+                                                      uint32_t& iy, uint32_t& iz) {
+  ix = cell_coord(x, P.origin[0], P.inv_cell, P.dims[0]);
+  iy = cell_coord(y, P.origin[1], P.inv_cell, P.dims[1]);
+  iz = cell_coord(z, P.origin[2], P.inv_cell, P.dims[2]);
+  return morton3(ix, iy, iz);
+}
+
+// Lower corner of a cell (local-coordinate anchor): origin + i * cell, two rounded ops.
+__host__ __device__ __forceinline__ float cell_corner(float origin, float cell, uint32_t i) {
+#pragma clang fp contract(off)
+  return origin + (float)i * cell;
+}
+
+// ---------------------------------------------------------------------------
+// applyForces + confineToBox (Forces.h:19-101) with the reference's FP32
+// operation order and no contraction:
+//   v.y += (1*g)*dt;  v += f*dt;  p = x + v*dt;  clamp p per axis to the
+//   enclosure inset by 0.001 (scaling that velocity component by the velocity
+//   damping); if anything clamped x += pdamp*(p - x);  p.w = 1.
+__host__ __device__ __forceinline__ void predict(const DevParams& P, float4& x, float4& v, const float4 f, float4& p) {
+#pragma clang fp contract(off)
+  const float dt = P.dt;
+  const float gdt = (1.0f * P.gravity) * dt;
+  v.y = v.y + gdt;
+  v.x = v.x + f.x * dt;
+  v.y = v.y + f.y * dt;
+  v.z = v.z + f.z * dt;
+  v.w = v.w + f.w * dt;
+  p.x = x.x + v.x * dt;
+  p.y = x.y + v.y * dt;
+  p.z = x.z + v.z * dt;
+  p.w = x.w + v.w * dt;
+  bool upd = false;
+  if (p.x < P.bmin[0]) {
+    v.x = P.vdamp * v.x;
+    p.x = P.bmin[0] + 0.001f;
+    upd = true;
+  } else if (p.x > P.bmax[0]) {
+    v.x = P.vdamp * v.x;
+    p.x = P.bmax[0] - 0.001f;
+    upd = true;
+  }
+  if (p.y < P.bmin[1]) {
+    v.y = P.vdamp * v.y;
+    p.y = P.bmin[1] + 0.001f;
+    upd = true;
+  } else if (p.y > P.bmax[1]) {
+    v.y = P.vdamp * v.y;
+    p.y = P.bmax[1] - 0.001f;
+    upd = true;
+  }
+  if (p.z < P.bmin[2]) {
+    v.z = P.vdamp * v.z;
+    p.z = P.bmin[2] + 0.001f;
+    upd = true;
+  } else if (p.z > P.bmax[2]) {
+    v.z = P.vdamp * v.z;
+    p.z = P.bmax[2] - 0.001f;
+    upd = true;
+  }
+  if (upd) {
+    x.x = x.x + P.pdamp * (p.x - x.x);
+    x.y = x.y + P.pdamp * (p.y - x.y);
+    x.z = x.z + P.pdamp * (p.z - x.z);
+    x.w = x.w + P.pdamp * (p.w - x.w);
+  }
+  p.w = 1.0f;
+}
+
+#ifdef __HIP_DEVICE_COMPILE__
+#define PBD_DEVICE_ONLY 1
+#endif
+
+// ---------------------------------------------------------------------------
+// Neighbour stencil walk (findContacts Collision.h:11-76, then findNeighbours
+// Collision.h:87-160): the 27 cells around the particle's cell in (x, y, z)
+// nested order -1..1, then the shells 2..kernelWidthNeighbours, every
+// particle j != s of each occupied cell in sorted order, stopping at the cap.
+// body(j, bx, by, bz, contact) receives the base vector b = q_s + (corner_s -
+// corner_j) so that p_s - p_j = b - q_j in cell-local coordinates.
+struct CellFrame {
+  uint32_t ix, iy, iz;
+  uint32_t kx, ky, kz;   // Morton components of the own cell
+  float cx, cy, cz;      // own corner
+};
+
+__device__ __forceinline__ CellFrame make_frame(const DevParams& P, uint32_t key) {
+  CellFrame f;
+  f.ix = compact10(key >> 2);
+  f.iy = compact10(key >> 1);
+  f.iz = compact10(key);
+  f.kx = key & kMortonX;
+  f.ky = key & kMortonY;
+  f.kz = key & kMortonZ;
+  f.cx = cell_corner(P.origin[0], P.cell, f.ix);
+  f.cy = cell_corner(P.origin[1], P.cell, f.iy);
+  f.cz = cell_corner(P.origin[2], P.cell, f.iz);
+  return f;
+}
+
+template <class Body>
+__device__ __forceinline__ void visit_cell(const DevParams& P, uint32_t s, uint32_t k, float bx, float by, float bz,
+                                           bool contact, const uint32_t* __restrict__ cs,
+                                           const uint32_t* __restrict__ ce, uint32_t& cnt, Body& body) {
+  const uint32_t st = cs[k];
+  if (st == 0xFFFFFFFFu) return;
+  const uint32_t en = ce[k];
+  for (uint32_t j = st; j < en; ++j) {
+    if (j == s) continue;
+    body(j, bx, by, bz, contact);
+    if (++cnt == P.cap) return;
+  }
+}
+
+// Returns the neighbour count; n_contacts receives the contact count.
+template <class Body>
+__device__ __forceinline__ uint32_t visit_neighbours(const DevParams& P, uint32_t s, const CellFrame& f, float qx,
+                                                     float qy, float qz, bool contacts, bool shells,
+                                                     const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+                                                     uint32_t& n_contacts, Body body) {
+  uint32_t cnt = 0;
+  const uint32_t cap = P.cap;
+  if (contacts) {
+    for (int dx = -1; dx <= 1; ++dx) {
+      const uint32_t nx = f.ix + (uint32_t)dx;
+      if (nx >= P.dims[0]) continue;
+      const uint32_t kx = dilated_add(f.kx, dx, kMortonX, 2);
+      const float bx = qx + (f.cx - cell_corner(P.origin[0], P.cell, nx));
+      for (int dy = -1; dy <= 1; ++dy) {
+        const uint32_t ny = f.iy + (uint32_t)dy;
+        if (ny >= P.dims[1]) continue;
+        const uint32_t ky = dilated_add(f.ky, dy, kMortonY, 1);
+        const float by = qy + (f.cy - cell_corner(P.origin[1], P.cell, ny));
+        for (int dz = -1; dz <= 1; ++dz) {
+          const uint32_t nz = f.iz + (uint32_t)dz;
+          if (nz >= P.dims[2]) continue;
+          const uint32_t kz = dilated_add(f.kz, dz, kMortonZ, 0);
+          const float bz = qz + (f.cz - cell_corner(P.origin[2], P.cell, nz));
+          visit_cell(P, s, kx | ky | kz, bx, by, bz, true, cs, ce, cnt, body);
+          if (cnt == cap) goto contacts_done;
+        }
+      }
+    }
+  }
+contacts_done:
+  n_contacts = cnt;
+  if (shells && cnt < cap) {
+    const int kw = P.kw;
+    for (int sh = 2; sh <= kw; ++sh) {
+      for (int dx = -sh; dx <= sh; ++dx) {
+        const uint32_t nx = f.ix + (uint32_t)dx;
+        if (nx >= P.dims[0]) continue;
+        const uint32_t kx = dilated_add(f.kx, dx, kMortonX, 2);
+        const float bx = qx + (f.cx - cell_corner(P.origin[0], P.cell, nx));
+        const bool ex = (dx == -sh || dx == sh);
+        for (int dy = -sh; dy <= sh; ++dy) {
+          const uint32_t ny = f.iy + (uint32_t)dy;
+          if (ny >= P.dims[1]) continue;
+          const uint32_t ky = dilated_add(f.ky, dy, kMortonY, 1);
+          const float by = qy + (f.cy - cell_corner(P.origin[1], P.cell, ny));
+          const bool ey = ex || (dy == -sh || dy == sh);
+          for (int dz = -sh; dz <= sh; ++dz) {
+            if (!ey && dz != -sh && dz != sh) continue;
+            const uint32_t nz = f.iz + (uint32_t)dz;
+            if (nz >= P.dims[2]) continue;
+            const uint32_t kz = dilated_add(f.kz, dz, kMortonZ, 0);
+            const float bz = qz + (f.cz - cell_corner(P.origin[2], P.cell, nz));
+            visit_cell(P, s, kx | ky | kz, bx, by, bz, false, cs, ce, cnt, body);
+            if (cnt == cap) return cnt;
+          }
+        }
+      }
+    }
+  }
+  return cnt;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel functions, unnormalised (coefficients wc / gc applied by the caller).
+//   reference (Density.h:14-42): W ~ ((h^2 - r^2)^3)^2 on 0.001 < r < h - 0.001;
+//                                grad ~ (h - r)^2 / r * r_vec on r > 1e-4, h - r > 1e-4
+//   textbook:                    W ~ (h^2 - r^2)^3 on r < h;  grad ~ (h - r)^2 / r * r_vec on 0 < r < h
+template <int V>
+__device__ __forceinline__ float w_unnorm(const DevParams& P, float r2) {
+  if (V == PBD_KERNEL_TEXTBOOK) {
+    if (r2 < P.hp2) {
+      const float t = P.hp2 - r2;
+      return t * t * t;
+    }
+    return 0.0f;
+  } else {
+    const float r = sqrtf(r2);
+    if (r > 0.001f && r < (P.hp - 0.001f)) {
+      const float t = P.hp2 - r2;
+      const float t3 = t * t * t;
+      return t3 * t3;
+    }
+    return 0.0f;
+  }
+}
+
+// grad magnitude factor g such that grad_unnorm = g * r_vec; also returns (h - r)^4 in hr4.
+template <int V>
+__device__ __forceinline__ float grad_factor(const DevParams& P, float r2, float& hr4) {
+  if (V == PBD_KERNEL_TEXTBOOK) {
+    if (r2 < P.hs2 && r2 > 0.0f) {
+      const float r = sqrtf(r2);
+      const float hr = P.hs - r;
+      const float hr2 = hr * hr;
+      hr4 = hr2 * hr2;
+      return hr2 / r;
+    }
+  } else {
+    const float r = sqrtf(r2);
+    if (r > 0.0001f && (P.hs - r) > 0.0001f) {
+      const float hr = P.hs - r;
+      const float hr2 = hr * hr;
+      hr4 = hr2 * hr2;
+      return hr2 / r;
+    }
+  }
+  hr4 = 0.0f;
+  return 0.0f;
+}
+
+__device__ __forceinline__ float ipow(float x, int n) {
+  bool inv = n < 0;
+  unsigned e = (unsigned)(inv ? -n : n);
+  float r = 1.0f;
+  while (e) {
+    if (e & 1u) r *= x;
+    x *= x;
+    e >>= 1;
+  }
+  return inv ? 1.0f / r : r;
+}
+
+}  // namespace pbd

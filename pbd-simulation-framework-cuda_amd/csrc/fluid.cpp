@@ -1,0 +1,503 @@
+// fluid.cpp - host orchestrator of the PBF step (Fluid::compute, Fluid.cu:14-97).
+#include "fluid.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <sstream>
+
+namespace pbd {
+
+const char* const kStageNames[ST_NUM] = {"predict_keys", "cell_reset", "sort",  "reorder", "lambda", "delta",
+                                         "collide",      "update",     "count", "omega",   "final"};
+
+Fluid::~Fluid() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  invalidate_graph();
+  for (auto& p : pending_) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : event_pool_) (void)hipEventDestroy(e);
+  for (void* p : allocations_) (void)hipFree(p);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int Fluid::fail(int code, const std::string& msg) {
+  status_ = code;
+  error_ = msg;
+  return code;
+}
+
+int Fluid::hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return PBD_OK;
+  std::ostringstream os;
+  os << what << ": " << hipGetErrorString(e);
+  return fail(e == hipErrorOutOfMemory ? PBD_ERR_NOMEM : PBD_ERR_HIP, os.str());
+}
+
+void Fluid::invalidate_graph() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+  graph_valid_ = false;
+}
+
+template <class T>
+static int dev_alloc(std::vector<void*>& list, T** p, size_t count) {
+  void* q = nullptr;
+  if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return PBD_ERR_NOMEM;
+  list.push_back(q);
+  *p = static_cast<T*>(q);
+  return PBD_OK;
+}
+
+int Fluid::alloc_table() {
+  const uint32_t ts = host_morton(params_.grid_dims[0] - 1u, params_.grid_dims[1] - 1u, params_.grid_dims[2] - 1u) + 1u;
+  if (B_.cstart && ts == B_.table_size) return PBD_OK;
+  if (B_.cstart) {
+    (void)hipFree(B_.cstart);
+    (void)hipFree(B_.cend);
+    allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), (void*)B_.cstart), allocations_.end());
+    allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), (void*)B_.cend), allocations_.end());
+    B_.cstart = B_.cend = nullptr;
+  }
+  if (dev_alloc(allocations_, &B_.cstart, ts) || dev_alloc(allocations_, &B_.cend, ts))
+    return fail(PBD_ERR_NOMEM, "cell table allocation failed");
+  B_.table_size = ts;
+  sort_.passes = sort_passes_for(ts);
+  return PBD_OK;
+}
+
+int Fluid::init(const pbd_params& p, uint32_t capacity) {
+  params_ = p;
+  const char* why = nullptr;
+  DevParams probe;
+  if (derive_device_params(p, 0, probe, &why) != PBD_OK) return fail(PBD_ERR_INVALID, why);
+  if (capacity == 0) capacity = p.texture_width * p.texture_width;
+  if (capacity == 0 || capacity >= (1u << 30)) return fail(PBD_ERR_INVALID, "capacity must be in [1, 2^30)");
+  cap_ = capacity;
+  tpb_ = p.threads_per_block ? p.threads_per_block : 256;
+  if (tpb_ < 64 || tpb_ > 1024 || (tpb_ % 64) != 0) tpb_ = 256;
+  if (hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate")) return status_;
+  const size_t c = cap_;
+  int rc = 0;
+  rc |= dev_alloc(allocations_, &B_.X, c);
+  rc |= dev_alloc(allocations_, &B_.P, c);
+  rc |= dev_alloc(allocations_, &B_.V, c);
+  rc |= dev_alloc(allocations_, &B_.C, c);
+  rc |= dev_alloc(allocations_, &B_.F, c);
+  rc |= dev_alloc(allocations_, &B_.Q[0], c);
+  rc |= dev_alloc(allocations_, &B_.Q[1], c);
+  rc |= dev_alloc(allocations_, &B_.QX[0], c);
+  rc |= dev_alloc(allocations_, &B_.QX[1], c);
+  rc |= dev_alloc(allocations_, &B_.VS, c);
+  rc |= dev_alloc(allocations_, &B_.CS, c);
+  rc |= dev_alloc(allocations_, &B_.DP, c);
+  rc |= dev_alloc(allocations_, &B_.OM, c);
+  rc |= dev_alloc(allocations_, &B_.LAM, c);
+  rc |= dev_alloc(allocations_, &B_.DENS, c);
+  rc |= dev_alloc(allocations_, &B_.NCNT, c);
+  rc |= dev_alloc(allocations_, &B_.CCNT, c);
+  rc |= dev_alloc(allocations_, &B_.keys_in, c);
+  sort_.n_max = cap_;
+  sort_.tiles_max = (cap_ + kSortTile - 1) / kSortTile;
+  rc |= dev_alloc(allocations_, &sort_.hist, 4 * 256);
+  rc |= dev_alloc(allocations_, &sort_.tile_ctr, 4);
+  rc |= dev_alloc(allocations_, &sort_.status, (size_t)4 * sort_.tiles_max * 256);
+  rc |= dev_alloc(allocations_, &sort_.err, 1);
+  rc |= dev_alloc(allocations_, &sort_.keys_alt, c);
+  rc |= dev_alloc(allocations_, &sort_.vals_alt, c);
+  rc |= dev_alloc(allocations_, &sort_.keys_out, c);
+  rc |= dev_alloc(allocations_, &sort_.vals_out, c);
+  rc |= dev_alloc(allocations_, &d_scratch_, 4);
+  if (rc) return fail(PBD_ERR_NOMEM, "device allocation failed");
+  if (alloc_table()) return status_;
+  // initializeDensity zeroes lambdas, deltaPositions, externalForces, omegas (Density.h:321-330)
+  hipError_t e = hipSuccess;
+  for (void* q : {(void*)B_.X, (void*)B_.P, (void*)B_.V, (void*)B_.C, (void*)B_.F, (void*)B_.DP, (void*)B_.OM,
+                  (void*)B_.VS, (void*)B_.CS})
+    if (e == hipSuccess) e = hipMemsetAsync(q, 0, c * 16, stream_);
+  for (void* q : {(void*)B_.LAM, (void*)B_.DENS, (void*)B_.NCNT, (void*)B_.CCNT})
+    if (e == hipSuccess) e = hipMemsetAsync(q, 0, c * 4, stream_);
+  if (e == hipSuccess) e = hipMemsetAsync(sort_.err, 0, 4, stream_);
+  if (e == hipSuccess) e = hipMemsetAsync(d_scratch_, 0, 16, stream_);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  if (hip_check(e, "initial zeroing")) return status_;
+  stage_ms_.assign(ST_NUM, 0.0);
+  stage_launches_.assign(ST_NUM, 0);
+  keys_sorted_ = sort_.keys_out;
+  ids_sorted_ = sort_.vals_out;
+  return PBD_OK;
+}
+
+int Fluid::set_params(const pbd_params& p) {
+  const char* why = nullptr;
+  DevParams probe;
+  if (derive_device_params(p, n_, probe, &why) != PBD_OK) return fail(PBD_ERR_INVALID, why);
+  if (hip_check(hipStreamSynchronize(stream_), "sync")) return status_;
+  params_ = p;
+  tpb_ = p.threads_per_block ? p.threads_per_block : 256;
+  if (tpb_ < 64 || tpb_ > 1024 || (tpb_ % 64) != 0) tpb_ = 256;
+  dev_valid_ = false;
+  invalidate_graph();
+  return alloc_table();
+}
+
+int Fluid::add_particles(uint32_t n, const float* pos4, const float* vel4, const float* col4) {
+  if (n == 0) return PBD_OK;
+  if (!pos4) return fail(PBD_ERR_INVALID, "positions are required");
+  if ((uint64_t)n_ + n > cap_) {
+    std::ostringstream os;
+    os << "capacity exceeded: " << n_ << " + " << n << " > " << cap_;
+    return fail(PBD_ERR_CAPACITY, os.str());
+  }
+  if (hip_check(hipStreamSynchronize(stream_), "sync")) return status_;
+  const size_t off = n_, bytes = (size_t)n * 16;
+  std::vector<float> zeros, white;
+  if (!vel4) {
+    zeros.assign((size_t)n * 4, 0.0f);
+    vel4 = zeros.data();
+  }
+  if (!col4) {
+    white.assign((size_t)n * 4, 1.0f);
+    col4 = white.data();
+  }
+  std::vector<float> f0((size_t)n * 4, 0.0f);
+  hipError_t e = hipMemcpy(B_.X + off, pos4, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(B_.P + off, pos4, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(B_.V + off, vel4, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(B_.C + off, col4, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(B_.F + off, f0.data(), bytes, hipMemcpyHostToDevice);
+  if (hip_check(e, "add_particles upload")) return status_;
+  n_ += n;
+  dev_valid_ = false;
+  invalidate_graph();
+  return PBD_OK;
+}
+
+int Fluid::clear_particles() {
+  if (hip_check(hipStreamSynchronize(stream_), "sync")) return status_;
+  n_ = 0;
+  dev_valid_ = false;
+  invalidate_graph();
+  return PBD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// timing
+void Fluid::stage_begin(int stage) {
+  if (!timing_) return;
+  (void)stage;
+  hipEvent_t a;
+  if (!event_pool_.empty()) {
+    a = event_pool_.back();
+    event_pool_.pop_back();
+  } else {
+    (void)hipEventCreate(&a);
+  }
+  (void)hipEventRecord(a, stream_);
+  cur_begin_ = a;
+}
+
+void Fluid::stage_end(int stage) {
+  if (!timing_ || !cur_begin_) return;
+  hipEvent_t b;
+  if (!event_pool_.empty()) {
+    b = event_pool_.back();
+    event_pool_.pop_back();
+  } else {
+    (void)hipEventCreate(&b);
+  }
+  (void)hipEventRecord(b, stream_);
+  pending_.push_back({stage, cur_begin_, b});
+  cur_begin_ = nullptr;
+}
+
+int Fluid::collect_timings() {
+  for (auto& p : pending_) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      stage_ms_[p.stage] += ms;
+      stage_launches_[p.stage] += 1;
+    }
+    event_pool_.push_back(p.a);
+    event_pool_.push_back(p.b);
+  }
+  pending_.clear();
+  return PBD_OK;
+}
+
+int Fluid::timings(const char** names, double* ms, uint64_t* launches, uint32_t cap, uint32_t* count) {
+  if (int rc = sync()) return rc;
+  for (uint32_t i = 0; i < (uint32_t)ST_NUM && i < cap; ++i) {
+    if (names) names[i] = kStageNames[i];
+    if (ms) ms[i] = stage_ms_[i];
+    if (launches) launches[i] = stage_launches_[i];
+  }
+  if (count) *count = ST_NUM;
+  return PBD_OK;
+}
+
+void Fluid::reset_timings() {
+  std::fill(stage_ms_.begin(), stage_ms_.end(), 0.0);
+  std::fill(stage_launches_.begin(), stage_launches_.end(), 0);
+}
+
+// ---------------------------------------------------------------------------
+// The step.
+#define PBD_LAUNCH(stage, call)                                   \
+  do {                                                            \
+    stage_begin(stage);                                           \
+    hipError_t e_ = (call);                                       \
+    stage_end(stage);                                             \
+    if (e_ != hipSuccess) return hip_check(e_, kStageNames[stage]); \
+  } while (0)
+
+int Fluid::enqueue_substep() {
+  const DevParams& P = dev_;
+  const hipStream_t st = stream_;
+  const uint32_t tpb = tpb_;
+  const uint32_t g = P.gates;
+  // 1-2. applyForces + initializeCellIds (Fluid.cu:21-25)
+  PBD_LAUNCH(ST_PREDICT, launch_predict_keys(st, tpb, P, B_));
+  // resetCellInfo (Fluid.cu:31): empty = {UINT_MAX, N}
+  PBD_LAUNCH(ST_CELL_RESET, launch_fill_u32(st, B_.cstart, 0xFFFFFFFFu, B_.table_size));
+  PBD_LAUNCH(ST_CELL_RESET, launch_fill_u32(st, B_.cend, P.n, B_.table_size));
+  // 3. sortIds (Fluid.cu:27)
+  PBD_LAUNCH(ST_SORT, launch_sort(st, sort_, P.n, B_.keys_in, &keys_sorted_, &ids_sorted_));
+  // 4-5. reorderStorage + computeCellInfo (Fluid.cu:29,33)
+  PBD_LAUNCH(ST_REORDER, launch_reorder(st, tpb, P, B_, keys_sorted_, ids_sorted_));
+  const uint32_t* keys = keys_sorted_;
+
+  // 6-7. solver loop (Fluid.cu:49-80); contacts/neighbours are walked inside the passes
+  const bool lambda_on = g & G_LAMBDA, delta_on = g & G_DELTA, apply_on = g & G_APPLY_DELTA;
+  const bool update_on = g & G_UPDATE;
+  const uint32_t S = params_.stabilization_iterations;
+  const uint32_t iters = params_.solver_iterations;
+  int cur = 0;
+  bool counted = false, updated = false;
+  for (uint32_t it = 0; it < iters; ++it) {
+    const bool last_it = it + 1 == iters;
+    if (lambda_on) {
+      PBD_LAUNCH(ST_LAMBDA, launch_lambda(st, tpb, P, B_, B_.Q[cur], keys, !counted));
+      counted = true;
+    }
+    // the Q-writing kernels of this iteration
+    struct K {
+      bool delta, collide;
+      uint32_t flags;
+    };
+    K ks[128];
+    uint32_t nk = 0;
+    const uint32_t ncoll = std::min<uint32_t>(S, 100);
+    if (delta_on) {
+      uint32_t fl = 0;
+      if (apply_on && ncoll <= 1) fl |= DF_APPLY;
+      if (ncoll >= 2 || !apply_on || last_it) fl |= DF_WRITE_DP;
+      ks[nk++] = {true, ncoll >= 1, fl};
+      for (uint32_t c = 1; c < ncoll; ++c) ks[nk++] = {false, true, (apply_on && c + 1 == ncoll) ? DF_APPLY : 0u};
+    } else {
+      for (uint32_t c = 0; c < ncoll; ++c) ks[nk++] = {false, true, (apply_on && c + 1 == ncoll) ? DF_APPLY : 0u};
+      if (ncoll == 0 && apply_on) ks[nk++] = {false, false, DF_APPLY};
+    }
+    for (uint32_t k = 0; k < nk; ++k) {
+      uint32_t fl = ks[k].flags;
+      if (last_it && k + 1 == nk && update_on) {
+        fl |= DF_UPDATE;
+        updated = true;
+      }
+      const int stage = ks[k].delta ? ST_DELTA : ST_COLLIDE;
+      PBD_LAUNCH(stage, launch_delta(st, tpb, P, B_, fl, ks[k].delta, ks[k].collide, B_.Q[cur], B_.QX[cur],
+                                     B_.Q[1 - cur], B_.QX[1 - cur], keys));
+      cur = 1 - cur;
+    }
+  }
+  if (!counted) PBD_LAUNCH(ST_COUNT, launch_count(st, tpb, P, B_, keys));
+  // 8. updatePositions (Fluid.cu:82-84) when not fused into the last solver pass
+  if (update_on && !updated) {
+    PBD_LAUNCH(ST_UPDATE, launch_update(st, tpb, P, B_, B_.Q[cur], B_.QX[cur]));
+    updated = true;
+  }
+  // 9. computeOmega (Fluid.cu:86-88)
+  if (g & G_OMEGA) PBD_LAUNCH(ST_OMEGA, launch_omega(st, tpb, P, B_, B_.Q[cur], keys));
+  // 10-11. computeVorticity + computeViscosity (Fluid.cu:90-96) and the output
+  uint32_t ff = 0;
+  if (g & G_VORTICITY) ff |= FF_VORT;
+  if (g & G_VISCOSITY) ff |= FF_XSPH;
+  if (updated) ff |= FF_UPDATED;
+  PBD_LAUNCH(ST_FINAL, launch_final(st, tpb, P, B_, ff, B_.Q[cur], B_.QX[cur], keys));
+  return PBD_OK;
+}
+
+void Fluid::compute() {
+  if (status_ != PBD_OK) return;
+  if (n_ == 0) return;  // Fluid.cu:19
+  if (params_.collision_type == 1) {
+    fail(PBD_ERR_UNSUPPORTED, "collisionType 1 is not available in this build");
+    return;
+  }
+  if (!dev_valid_) {
+    const char* why = nullptr;
+    if (derive_device_params(params_, n_, dev_, &why) != PBD_OK) {
+      fail(PBD_ERR_INVALID, why);
+      return;
+    }
+    dev_valid_ = true;
+  }
+  enqueue_substep();
+}
+
+int Fluid::step(uint32_t frames, bool graph) {
+  if (status_ != PBD_OK) return status_;
+  const uint32_t sub = params_.substeps ? params_.substeps : 1;
+  if (n_ == 0) return PBD_OK;
+  if (!graph || timing_) {
+    for (uint32_t f = 0; f < frames; ++f)
+      for (uint32_t s = 0; s < sub; ++s) {
+        compute();
+        if (status_ != PBD_OK) return status_;
+      }
+    return PBD_OK;
+  }
+  if (params_.collision_type == 1) return fail(PBD_ERR_UNSUPPORTED, "collisionType 1 is not available in this build");
+  if (!graph_valid_) {
+    if (!dev_valid_) {
+      const char* why = nullptr;
+      if (derive_device_params(params_, n_, dev_, &why) != PBD_OK) return fail(PBD_ERR_INVALID, why);
+      dev_valid_ = true;
+    }
+    if (hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+      return status_;
+    int rc = enqueue_substep();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(stream_, &g);
+    if (rc != PBD_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (hip_check(e, "hipStreamEndCapture")) return status_;
+    graph_ = g;
+    if (hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate"))
+      return status_;
+    graph_valid_ = true;
+  }
+  for (uint32_t f = 0; f < frames; ++f)
+    for (uint32_t s = 0; s < sub; ++s)
+      if (hip_check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch")) return status_;
+  return PBD_OK;
+}
+
+int Fluid::sync() {
+  if (hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize")) return status_;
+  if (timing_ || !pending_.empty()) collect_timings();
+  uint32_t err = 0;
+  if (hip_check(hipMemcpy(&err, sort_.err, 4, hipMemcpyDeviceToHost), "read sort flag")) return status_;
+  if (err) return fail(PBD_ERR_HIP, "sort look-back exceeded its spin limit");
+  return status_;
+}
+
+// ---------------------------------------------------------------------------
+// buffers
+static int elem_info(int which, size_t* elem, bool* table) {
+  *table = false;
+  switch (which) {
+    case PBD_BUF_POSITIONS:
+    case PBD_BUF_PREDICTED:
+    case PBD_BUF_VELOCITIES:
+    case PBD_BUF_COLORS:
+    case PBD_BUF_EXTERNAL_FORCES:
+    case PBD_BUF_OMEGAS:
+    case PBD_BUF_DELTA_POSITIONS:
+      *elem = 16;
+      return PBD_OK;
+    case PBD_BUF_DENSITIES:
+    case PBD_BUF_LAMBDAS:
+    case PBD_BUF_CELL_IDS:
+    case PBD_BUF_PARTICLE_IDS:
+    case PBD_BUF_NEIGHBOUR_COUNTERS:
+    case PBD_BUF_CONTACT_COUNTERS:
+      *elem = 4;
+      return PBD_OK;
+    case PBD_BUF_CELL_STARTS:
+    case PBD_BUF_CELL_ENDINGS:
+      *elem = 4;
+      *table = true;
+      return PBD_OK;
+    default:
+      return PBD_ERR_INVALID;
+  }
+}
+
+int Fluid::buffer_bytes(int which, size_t* bytes) const {
+  size_t elem;
+  bool table;
+  if (elem_info(which, &elem, &table)) return PBD_ERR_INVALID;
+  *bytes = elem * (table ? B_.table_size : n_);
+  return PBD_OK;
+}
+
+int Fluid::download(int which, void* dst, size_t bytes) {
+  size_t need;
+  if (buffer_bytes(which, &need) || bytes > need || (bytes && !dst))
+    return fail(PBD_ERR_INVALID, "download: bad buffer id or size");
+  if (int rc = sync()) return rc;
+  const void* src = nullptr;
+  switch (which) {
+    case PBD_BUF_POSITIONS: src = B_.X; break;
+    case PBD_BUF_PREDICTED: src = B_.P; break;
+    case PBD_BUF_VELOCITIES: src = B_.V; break;
+    case PBD_BUF_COLORS: src = B_.C; break;
+    case PBD_BUF_EXTERNAL_FORCES: src = B_.F; break;
+    case PBD_BUF_DENSITIES: src = B_.DENS; break;
+    case PBD_BUF_LAMBDAS: src = B_.LAM; break;
+    case PBD_BUF_CELL_IDS: src = keys_sorted_; break;
+    case PBD_BUF_PARTICLE_IDS: src = ids_sorted_; break;
+    case PBD_BUF_CELL_STARTS: src = B_.cstart; break;
+    case PBD_BUF_CELL_ENDINGS: src = B_.cend; break;
+    case PBD_BUF_NEIGHBOUR_COUNTERS: src = B_.NCNT; break;
+    case PBD_BUF_CONTACT_COUNTERS: src = B_.CCNT; break;
+    case PBD_BUF_OMEGAS: src = B_.OM; break;
+    case PBD_BUF_DELTA_POSITIONS: src = B_.DP; break;
+  }
+  if (bytes == 0) return PBD_OK;
+  return hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "download");
+}
+
+int Fluid::upload(int which, const void* src, size_t bytes) {
+  void* dst = nullptr;
+  switch (which) {
+    case PBD_BUF_POSITIONS: dst = B_.X; break;
+    case PBD_BUF_PREDICTED: dst = B_.P; break;
+    case PBD_BUF_VELOCITIES: dst = B_.V; break;
+    case PBD_BUF_COLORS: dst = B_.C; break;
+    case PBD_BUF_EXTERNAL_FORCES: dst = B_.F; break;
+    default: return fail(PBD_ERR_INVALID, "upload: only particle state buffers can be uploaded");
+  }
+  if (bytes > (size_t)n_ * 16 || (bytes && !src)) return fail(PBD_ERR_INVALID, "upload: size exceeds particle count");
+  if (int rc = sync()) return rc;
+  if (bytes == 0) return PBD_OK;
+  return hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "upload");
+}
+
+int Fluid::check_finite(uint32_t* bad) {
+  if (hip_check(hipMemsetAsync(d_scratch_, 0, 4, stream_), "memset")) return status_;
+  if (hip_check(launch_check_finite(stream_, n_, B_.X, B_.V, d_scratch_), "check_finite")) return status_;
+  uint32_t b = 0;
+  if (hip_check(hipMemcpyAsync(&b, d_scratch_, 4, hipMemcpyDeviceToHost, stream_), "copy")) return status_;
+  if (hip_check(hipStreamSynchronize(stream_), "sync")) return status_;
+  if (bad) *bad = b;
+  return b ? PBD_ERR_NONFINITE : PBD_OK;
+}
+
+void Fluid::get_buffers(pbd_buffers* out) const {
+  out->positions4 = B_.X;
+  out->predicted_positions4 = B_.P;
+  out->velocities4 = B_.V;
+  out->colors4 = B_.C;
+  out->external_forces4 = B_.F;
+  out->num_particles = n_;
+  out->capacity = cap_;
+}
+
+}  // namespace pbd

@@ -1,0 +1,131 @@
+// fluid.hpp - host orchestrator of the PBF step.
+//
+// Mirrors the reference's step interface: Cuda_Computable::compute()
+// (src/cuda/Cuda_Computable.h:8-18) implemented by Fluid (src/app/simulation/
+// Fluid.h, Fluid.cu:4-97).  The constructor's allocation replaces
+// cudaInitializeKernels (kernel/Initialize.h:122-130) and the GL-owned particle
+// buffers of Particles (src/app/rendering/Particles.cc:26-143); compute() runs
+// the stages of Fluid.cu:14-97 on one HIP stream with the same Active.* gates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "params.hpp"
+#include "pbd/pbd.h"
+
+namespace pbd {
+
+class Cuda_Computable {
+ public:
+  virtual ~Cuda_Computable() = default;
+  virtual void compute() = 0;
+};
+
+class Fluid : public Cuda_Computable {
+ public:
+  Fluid() = default;
+  ~Fluid() override;
+  Fluid(const Fluid&) = delete;
+  Fluid& operator=(const Fluid&) = delete;
+
+  // Allocate for `capacity` particles on the current device.
+  int init(const pbd_params& p, uint32_t capacity);
+  int set_params(const pbd_params& p);
+  const pbd_params& params() const { return params_; }
+
+  // One substep (Fluid::compute).  Errors are kept in status().
+  void compute() override;
+  // frames x substeps calls of compute(); graph = replay a captured substep.
+  int step(uint32_t frames, bool graph);
+  int sync();
+
+  int add_particles(uint32_t n, const float* pos4, const float* vel4, const float* col4);
+  int clear_particles();
+  uint32_t num_particles() const { return n_; }
+  uint32_t capacity() const { return cap_; }
+
+  int download(int which, void* dst, size_t bytes);
+  int upload(int which, const void* src, size_t bytes);
+  int buffer_bytes(int which, size_t* bytes) const;
+  int check_finite(uint32_t* bad);
+  void get_buffers(pbd_buffers* out) const;
+
+  void set_timing(bool on) { timing_ = on; }
+  int timings(const char** names, double* ms, uint64_t* launches, uint32_t cap, uint32_t* count);
+  void reset_timings();
+
+  int status() const { return status_; }
+  const std::string& error() const { return error_; }
+  void clear_status() {
+    status_ = PBD_OK;
+    error_.clear();
+  }
+
+ private:
+  int fail(int code, const std::string& msg);
+  int hip_check(hipError_t e, const char* what);
+  int alloc_table();
+  int enqueue_substep();
+  void stage_begin(int stage);
+  void stage_end(int stage);
+  int collect_timings();
+  void invalidate_graph();
+
+  pbd_params params_{};
+  DevParams dev_{};
+  bool dev_valid_ = false;
+  uint32_t n_ = 0;
+  uint32_t cap_ = 0;
+  uint32_t tpb_ = 256;
+  hipStream_t stream_ = nullptr;
+  StepBuffers B_{};
+  SortPlan sort_{};
+  const uint32_t* keys_sorted_ = nullptr;
+  const uint32_t* ids_sorted_ = nullptr;
+  uint32_t* d_scratch_ = nullptr;  // small device scratch: [0] finite count, [1] unused
+
+  // graph of one substep
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  bool graph_valid_ = false;
+
+  // timing
+  bool timing_ = false;
+  struct Pending {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> event_pool_;
+  hipEvent_t cur_begin_ = nullptr;
+  std::vector<double> stage_ms_;
+  std::vector<uint64_t> stage_launches_;
+
+  int status_ = PBD_OK;
+  std::string error_;
+  std::vector<void*> allocations_;
+};
+
+// Stage names used by the timers.
+enum Stage {
+  ST_PREDICT = 0,
+  ST_CELL_RESET,
+  ST_SORT,
+  ST_REORDER,
+  ST_LAMBDA,
+  ST_DELTA,
+  ST_COLLIDE,
+  ST_UPDATE,
+  ST_COUNT,
+  ST_OMEGA,
+  ST_FINAL,
+  ST_NUM
+};
+extern const char* const kStageNames[ST_NUM];
+
+}  // namespace pbd

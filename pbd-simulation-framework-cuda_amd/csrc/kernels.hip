@@ -1,0 +1,492 @@
+// kernels.hip - gfx950 kernels of the PBF step (everything but the sort).
+//
+// Data layout during a step (all arrays in sorted order, float4 = 16 B):
+//   Q   (p* - corner(cell), p*.w)   predicted position in cell-local coordinates
+//   QX  (x  - corner(cell), x.w)    position in cell-local coordinates
+//   VS  velocity
+//   LAM lambda (float), DP delta p (float4), OM omega (float4)
+// Working in cell-local coordinates keeps every FP32 position update at the
+// resolution of the cell instead of the world box: the Jacobi updates
+// p* +- dp round at ulp(0.1) rather than ulp(12.8).  The public buffers
+// positions4 / predictedPositions4 / velocities4 / colors4 are read once by the
+// reorder gather and written once by the final kernel.
+//
+// Reference kernels restated (src/app/simulation/kernel/):
+//   k_predict_keys  applyForces + initializeCellIds   Forces.h:65-101, SortReorder.h:13-28
+//   k_reorder       copy + reorder + computeCellInfo   SortReorder.h:53-129,160-188
+//   k_lambda        computeLambda (+ neighbour counts, density)  Density.h:48-93, Collision.h:11-160
+//   k_delta         computeDeltaPositions + solveCollisions + applyDeltaPositions (+ updatePositions)
+//                   Density.h:104-168, Collision.h:171-285, Update.h:11-32
+//   k_omega         computeOmega                       Density.h:232-269
+//   k_final         computeVorticity + computeViscosity + output   Density.h:179-221,280-310
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace pbd {
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint32_t n) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// applyForces + initializeCellIds, fused.  The predicted state itself is not
+// stored: the reorder gather recomputes it with the same FP32 operations.
+__global__ void k_predict_keys(DevParams P, const float4* __restrict__ X, const float4* __restrict__ V,
+                               const float4* __restrict__ F, const float4* __restrict__ Pold,
+                               uint32_t* __restrict__ keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  float4 p;
+  if (P.gates & G_APPLY_FORCES) {
+    float4 x = X[i], v = V[i];
+    predict(P, x, v, F[i], p);
+  } else {
+    p = Pold[i];
+  }
+  uint32_t ix, iy, iz;
+  keys[i] = cell_key(P, p.x, p.y, p.z, ix, iy, iz);
+}
+
+// ---------------------------------------------------------------------------
+// Reorder gather + cell ranges.  Cell boundaries are found per wavefront: each
+// lane compares its key with the previous lane's (DPP/bpermute shuffle), lane 0
+// with the previous wave's last key.
+__global__ void k_reorder(DevParams P, const uint32_t* __restrict__ keys_s, const uint32_t* __restrict__ ids_s,
+                          const float4* __restrict__ X, const float4* __restrict__ V, const float4* __restrict__ F,
+                          const float4* __restrict__ Pold, const float4* __restrict__ C, float4* __restrict__ Q,
+                          float4* __restrict__ QX, float4* __restrict__ VS, float4* __restrict__ CS,
+                          uint32_t* __restrict__ cstart, uint32_t* __restrict__ cend) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const uint32_t key = keys_s[s];
+  uint32_t prev = __shfl_up(key, 1);
+  if ((threadIdx.x & (warpSize - 1)) == 0) prev = s ? keys_s[s - 1] : 0xFFFFFFFFu;
+  if (key != prev) {
+    cstart[key] = s;
+    if (s) cend[prev] = s;
+  }
+  uint32_t j = ids_s[s];
+  if (j >= P.n) j = P.n - 1;  // ids are a permutation of [0, n); the clamp only bounds a failed sort
+  float4 x = X[j], v = V[j], p;
+  if (P.gates & G_APPLY_FORCES) {
+    predict(P, x, v, F[j], p);
+  } else {
+    p = Pold[j];
+  }
+  uint32_t ix, iy, iz;
+  cell_key(P, p.x, p.y, p.z, ix, iy, iz);
+  const float cx = cell_corner(P.origin[0], P.cell, ix);
+  const float cy = cell_corner(P.origin[1], P.cell, iy);
+  const float cz = cell_corner(P.origin[2], P.cell, iz);
+  Q[s] = make_float4(p.x - cx, p.y - cy, p.z - cz, p.w);
+  QX[s] = make_float4(x.x - cx, x.y - cy, x.z - cz, x.w);
+  VS[s] = v;
+  if (!(P.gates & G_FIND_NEIGHBOURS)) CS[s] = C[j];
+}
+
+// ---------------------------------------------------------------------------
+// computeLambda.  FIRST also writes the neighbour / contact counts
+// (findContacts + findNeighbours) and the density of the pass.
+template <int VAR, bool FIRST>
+__global__ void k_lambda(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
+                         const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce, float* __restrict__ lam,
+                         float* __restrict__ dens, uint32_t* __restrict__ ncnt, uint32_t* __restrict__ ccnt) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const float4 qi = Q[s];
+  const CellFrame f = make_frame(P, keys[s]);
+  float A = 0.0f, B = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  uint32_t nc = 0;
+  const uint32_t cnt = visit_neighbours(
+      P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0, (P.gates & G_FIND_NEIGHBOURS) != 0, cs, ce, nc,
+      [&](uint32_t j, float bx, float by, float bz, bool) {
+        const float4 qj = Q[j];
+        const float rx = bx - qj.x, ry = by - qj.y, rz = bz - qj.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        A += w_unnorm<VAR>(P, r2);
+        float hr4;
+        const float g = grad_factor<VAR>(P, r2, hr4);
+        gx += g * rx;
+        gy += g * ry;
+        gz += g * rz;
+        B += hr4;
+      });
+  const float W = P.wc * A + P.w0;
+  const float G2 = P.gc * P.gc * (gx * gx + gy * gy + gz * gz);
+  const float B2 = P.gc * P.gc * B;
+  float l;
+  if (VAR == PBD_KERNEL_TEXTBOOK) {
+    l = -(W * P.inv_rho0 - 1.0f) / ((G2 + B2) * P.inv_rho0_sq + P.eps);
+  } else {
+    l = -(W * P.inv_rho0 - 1.0f) / (B2 * P.inv_rho0 + G2 * P.inv_rho0 + P.eps);
+  }
+  lam[s] = l;
+  if (FIRST) {
+    dens[s] = W;
+    ncnt[s] = cnt;
+    ccnt[s] = nc;
+  }
+}
+
+// Neighbour / contact counts only (when computeLambda does not run first).
+__global__ void k_count(DevParams P, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
+                        const uint32_t* __restrict__ ce, uint32_t* __restrict__ ncnt, uint32_t* __restrict__ ccnt) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const CellFrame f = make_frame(P, keys[s]);
+  uint32_t nc = 0;
+  const uint32_t cnt =
+      visit_neighbours(P, s, f, 0.0f, 0.0f, 0.0f, (P.gates & G_FIND_CONTACTS) != 0,
+                       (P.gates & G_FIND_NEIGHBOURS) != 0, cs, ce, nc, [&](uint32_t, float, float, float, bool) {});
+  ncnt[s] = cnt;
+  ccnt[s] = nc;
+}
+
+// ---------------------------------------------------------------------------
+// computeDeltaPositions (DELTA) + one collision pass (COLLIDE, gather over the
+// contact cells with the pre-pass state: Jacobi) + applyDeltaPositions
+// (flags & DF_APPLY) + updatePositions (flags & DF_UPDATE).  Reads Q/QX,
+// writes Qo/QXo (ping-pong).  Without DELTA the stored DP is applied.
+template <int VAR, bool DELTA, bool COLLIDE>
+__global__ void k_delta(DevParams P, uint32_t flags, const float4* __restrict__ Q, const float4* __restrict__ QX,
+                        const float* __restrict__ lam, const uint32_t* __restrict__ keys,
+                        const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce, float4* __restrict__ Qo,
+                        float4* __restrict__ QXo, float4* __restrict__ DP, float4* __restrict__ VS) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const float4 qi = Q[s];
+  const CellFrame f = make_frame(P, keys[s]);
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  float px = 0.0f, py = 0.0f, pz = 0.0f;
+  if (DELTA || COLLIDE) {
+    const float li = DELTA ? lam[s] : 0.0f;
+    uint32_t nc = 0;
+    visit_neighbours(P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0,
+                     DELTA && (P.gates & G_FIND_NEIGHBOURS) != 0, cs, ce, nc,
+                     [&](uint32_t j, float bx, float by, float bz, bool contact) {
+                       const float4 qj = Q[j];
+                       const float rx = bx - qj.x, ry = by - qj.y, rz = bz - qj.z;
+                       const float r2 = rx * rx + ry * ry + rz * rz;
+                       if (DELTA) {
+                         const float w = P.wc * w_unnorm<VAR>(P, r2);
+                         const float sc = -P.k_scorr * ipow(w * P.inv_w_dq, P.n_scorr);
+                         float hr4;
+                         const float g = grad_factor<VAR>(P, r2, hr4);
+                         const float t = (li + lam[j] + sc) * g;
+                         ax += t * rx;
+                         ay += t * ry;
+                         az += t * rz;
+                       }
+                       if (COLLIDE && contact) {
+                         const float len = sqrtf(r2);
+                         const float ov = P.diameter - len;
+                         if (ov > 0.0f && len > 0.00001f) {
+                           // push along -(p_j - p_i) = r: coll_scale * ov * (p_j - p_i)/len
+                           const float t = -P.coll_scale * ov / len;
+                           px += t * rx;
+                           py += t * ry;
+                           pz += t * rz;
+                         }
+                       }
+                     });
+  }
+  float dx, dy, dz;
+  if (DELTA) {
+    const float sc = P.gc * P.inv_rho0;
+    dx = ax * sc;
+    dy = ay * sc;
+    dz = az * sc;
+    if (flags & DF_WRITE_DP) DP[s] = make_float4(dx, dy, dz, 0.0f);
+  } else {
+    const float4 d = (flags & DF_APPLY) ? DP[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    dx = d.x;
+    dy = d.y;
+    dz = d.z;
+  }
+  const float4 qxi = QX[s];
+  float nx = qi.x + px, ny = qi.y + py, nz = qi.z + pz;
+  float4 qx = make_float4(qxi.x + px, qxi.y + py, qxi.z + pz, qxi.w);
+  if (flags & DF_APPLY) {
+    if (VAR == PBD_KERNEL_REFERENCE) {
+      nx -= dx;
+      ny -= dy;
+      nz -= dz;
+    } else {
+      nx += dx;
+      ny += dy;
+      nz += dz;
+      nx = fminf(fmaxf(nx, P.lo[0] - f.cx), P.hi[0] - f.cx);
+      ny = fminf(fmaxf(ny, P.lo[1] - f.cy), P.hi[1] - f.cy);
+      nz = fminf(fmaxf(nz, P.lo[2] - f.cz), P.hi[2] - f.cz);
+    }
+  }
+  if (flags & DF_UPDATE) {
+    VS[s] = make_float4((nx - qx.x) / P.dt, (ny - qx.y) / P.dt, (nz - qx.z) / P.dt, (qi.w - qx.w) / P.dt);
+    qx = make_float4(nx, ny, nz, qi.w);
+  }
+  Qo[s] = make_float4(nx, ny, nz, qi.w);
+  QXo[s] = qx;
+}
+
+// updatePositions alone (Update.h:11-32): v = (p* - x)/dt, x = p*.
+__global__ void k_update(DevParams P, const float4* __restrict__ Q, float4* __restrict__ QX, float4* __restrict__ VS) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const float4 q = Q[s], x = QX[s];
+  VS[s] = make_float4((q.x - x.x) / P.dt, (q.y - x.y) / P.dt, (q.z - x.z) / P.dt, (q.w - x.w) / P.dt);
+  QX[s] = q;
+}
+
+// ---------------------------------------------------------------------------
+// computeOmega: reference omega_i = sum (v_j - v_i) x grad(p_i - p_j);
+// textbook omega_i = (1/rho0) sum v_ij x grad_j W_ij = -(1/rho0) sum v_ij x grad_i W_ij
+// (SPH curl with the particle volume m/rho0, mass 1).
+template <int VAR>
+__global__ void k_omega(DevParams P, const float4* __restrict__ Q, const float4* __restrict__ VS,
+                        const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
+                        const uint32_t* __restrict__ ce, float4* __restrict__ OM) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const float4 qi = Q[s], vi = VS[s];
+  const CellFrame f = make_frame(P, keys[s]);
+  float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+  uint32_t nc = 0;
+  visit_neighbours(P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0, (P.gates & G_FIND_NEIGHBOURS) != 0, cs,
+                   ce, nc, [&](uint32_t j, float bx, float by, float bz, bool) {
+                     const float4 qj = Q[j];
+                     const float rx = bx - qj.x, ry = by - qj.y, rz = bz - qj.z;
+                     const float r2 = rx * rx + ry * ry + rz * rz;
+                     float hr4;
+                     const float g = grad_factor<VAR>(P, r2, hr4);
+                     const float4 vj = VS[j];
+                     const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
+                     const float gx = g * rx, gy = g * ry, gz = g * rz;
+                     ox += vy * gz - vz * gy;
+                     oy += vz * gx - vx * gz;
+                     oz += vx * gy - vy * gx;
+                   });
+  const float sc = (VAR == PBD_KERNEL_TEXTBOOK) ? -P.gc * P.vol : P.gc;
+  OM[s] = make_float4(ox * sc, oy * sc, oz * sc, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// computeVorticity (flags & FF_VORT) + computeViscosity (flags & FF_XSPH,
+// Jacobi: reads the pre-XSPH velocities) + output of the public buffers.
+// Textbook XSPH is Monaghan's form v_i += c (1/rho0) sum v_ij W_ij.
+template <int VAR>
+__global__ void k_final(DevParams P, uint32_t flags, const float4* __restrict__ Q, const float4* __restrict__ QX,
+                        const float4* __restrict__ VS, const float4* __restrict__ OM,
+                        const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
+                        const uint32_t* __restrict__ ce, const float4* __restrict__ CS,
+                        const uint32_t* __restrict__ ncnt, float4* __restrict__ Xo, float4* __restrict__ Po,
+                        float4* __restrict__ Vo, float4* __restrict__ Co, float4* __restrict__ Fo) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const float4 qi = Q[s], vi = VS[s];
+  const CellFrame f = make_frame(P, keys[s]);
+  const bool vort = (flags & FF_VORT) != 0, xsph = (flags & FF_XSPH) != 0;
+  float4 oi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (vort) oi = OM[s];
+  const float oil = sqrtf(oi.x * oi.x + oi.y * oi.y + oi.z * oi.z);
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f, sw = 0.0f;  // XSPH sum
+  float ex = 0.0f, ey = 0.0f, ez = 0.0f;             // vorticity gradient
+  if (vort || xsph) {
+    uint32_t nc = 0;
+    visit_neighbours(P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0,
+                     (P.gates & G_FIND_NEIGHBOURS) != 0, cs, ce, nc,
+                     [&](uint32_t j, float bx, float by, float bz, bool) {
+                       const float4 qj = Q[j];
+                       const float rx = bx - qj.x, ry = by - qj.y, rz = bz - qj.z;
+                       const float r2 = rx * rx + ry * ry + rz * rz;
+                       if (xsph) {
+                         const float w = w_unnorm<VAR>(P, r2);
+                         const float4 vj = VS[j];
+                         sx += (vj.x - vi.x) * w;
+                         sy += (vj.y - vi.y) * w;
+                         sz += (vj.z - vi.z) * w;
+                         sw += (vj.w - vi.w) * w;
+                       }
+                       if (vort) {
+                         const float4 oj = OM[j];
+                         if (VAR == PBD_KERNEL_REFERENCE) {
+                           const float dx = oj.x - oi.x, dy = oj.y - oi.y, dz = oj.z - oi.z;
+                           const float ol = sqrtf(dx * dx + dy * dy + dz * dz);
+                           ex += ol / (-rx + 0.00000001f);
+                           ey += ol / (-ry + 0.00000001f);
+                           ez += ol / (-rz + 0.00000001f);
+                         } else {
+                           float hr4;
+                           const float g = grad_factor<VAR>(P, r2, hr4);
+                           const float d = sqrtf(oj.x * oj.x + oj.y * oj.y + oj.z * oj.z) - oil;
+                           ex += d * g * rx;
+                           ey += d * g * ry;
+                           ez += d * g * rz;
+                         }
+                       }
+                     });
+  }
+  if (vort) {
+    float nx, ny, nz;
+    if (VAR == PBD_KERNEL_REFERENCE) {
+      const float EPS = 0.00000001f;
+      const float inv = 1.0f / (sqrtf(ex * ex + ey * ey + ez * ez) + EPS);
+      nx = inv * (ex + EPS);
+      ny = inv * (ey + EPS);
+      nz = inv * (ez + EPS);
+    } else {
+      // eta carries the factor gc; its direction is sign(gc) * eta
+      const float sg = P.gc < 0.0f ? -1.0f : 1.0f;
+      const float el = sqrtf(ex * ex + ey * ey + ez * ez);
+      const float inv = el > 0.0f ? sg / el : 0.0f;
+      nx = ex * inv;
+      ny = ey * inv;
+      nz = ez * inv;
+    }
+    Fo[s] = make_float4(P.e_vort * (ny * oi.z - nz * oi.y), P.e_vort * (nz * oi.x - nx * oi.z),
+                        P.e_vort * (nx * oi.y - ny * oi.x), 0.0f);
+  }
+  float4 v = vi;
+  if (xsph) {
+    const float c = P.c_visc * P.wc * P.vol;
+    v = make_float4(vi.x + c * sx, vi.y + c * sy, vi.z + c * sz, vi.w + c * sw);
+  }
+  const float4 xl = (flags & FF_UPDATED) ? qi : QX[s];
+  Xo[s] = make_float4(xl.x + f.cx, xl.y + f.cy, xl.z + f.cz, xl.w);
+  Po[s] = make_float4(qi.x + f.cx, qi.y + f.cy, qi.z + f.cz, qi.w);
+  Vo[s] = v;
+  if (P.gates & G_FIND_NEIGHBOURS) {
+    const float q = (float)ncnt[s] / (float)P.cap;
+    Co[s] = make_float4(q, q, q, 1.0f);
+  } else {
+    Co[s] = CS[s];
+  }
+}
+
+// Finite check of positions and velocities (failure detection).
+__global__ void k_check_finite(uint32_t n, const float4* __restrict__ X, const float4* __restrict__ V,
+                               uint32_t* __restrict__ bad) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 x = X[i], v = V[i];
+  const bool ok = isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) && isfinite(v.y) && isfinite(v.z);
+  if (!ok) atomicAdd(bad, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+static inline uint32_t blocks_for(uint32_t n, uint32_t tpb) { return (n + tpb - 1) / tpb; }
+
+hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n) {
+  if (n == 0) return hipSuccess;
+  uint32_t b = blocks_for(n, 256);
+  if (b > 4096) b = 4096;
+  hipLaunchKernelGGL(k_fill_u32, dim3(b), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_predict_keys(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B) {
+  hipLaunchKernelGGL(k_predict_keys, dim3(blocks_for(P.n, tpb)), dim3(tpb), 0, st, P, B.X, B.V, B.F, B.P, B.keys_in);
+  return hipGetLastError();
+}
+
+hipError_t launch_reorder(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,
+                          const uint32_t* keys_s, const uint32_t* ids_s) {
+  hipLaunchKernelGGL(k_reorder, dim3(blocks_for(P.n, tpb)), dim3(tpb), 0, st, P, keys_s, ids_s, B.X, B.V, B.F, B.P,
+                     B.C, B.Q[0], B.QX[0], B.VS, B.CS, B.cstart, B.cend);
+  return hipGetLastError();
+}
+
+hipError_t launch_lambda(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, const float4* Q,
+                         const uint32_t* keys_s, bool first) {
+  dim3 g(blocks_for(P.n, tpb)), b(tpb);
+  if (P.variant == PBD_KERNEL_TEXTBOOK) {
+    if (first)
+      hipLaunchKernelGGL((k_lambda<PBD_KERNEL_TEXTBOOK, true>), g, b, 0, st, P, Q, keys_s, B.cstart, B.cend, B.LAM,
+                         B.DENS, B.NCNT, B.CCNT);
+    else
+      hipLaunchKernelGGL((k_lambda<PBD_KERNEL_TEXTBOOK, false>), g, b, 0, st, P, Q, keys_s, B.cstart, B.cend, B.LAM,
+                         B.DENS, B.NCNT, B.CCNT);
+  } else {
+    if (first)
+      hipLaunchKernelGGL((k_lambda<PBD_KERNEL_REFERENCE, true>), g, b, 0, st, P, Q, keys_s, B.cstart, B.cend, B.LAM,
+                         B.DENS, B.NCNT, B.CCNT);
+    else
+      hipLaunchKernelGGL((k_lambda<PBD_KERNEL_REFERENCE, false>), g, b, 0, st, P, Q, keys_s, B.cstart, B.cend, B.LAM,
+                         B.DENS, B.NCNT, B.CCNT);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_count(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,
+                        const uint32_t* keys_s) {
+  hipLaunchKernelGGL(k_count, dim3(blocks_for(P.n, tpb)), dim3(tpb), 0, st, P, keys_s, B.cstart, B.cend, B.NCNT,
+                     B.CCNT);
+  return hipGetLastError();
+}
+
+template <int VAR>
+static void delta_dispatch(hipStream_t st, dim3 g, dim3 b, const DevParams& P, uint32_t flags, bool delta,
+                           bool collide, const float4* Q, const float4* QX, const float* lam, const uint32_t* keys,
+                           const uint32_t* cs, const uint32_t* ce, float4* Qo, float4* QXo, float4* DP, float4* VS) {
+  if (delta && collide)
+    hipLaunchKernelGGL((k_delta<VAR, true, true>), g, b, 0, st, P, flags, Q, QX, lam, keys, cs, ce, Qo, QXo, DP, VS);
+  else if (delta)
+    hipLaunchKernelGGL((k_delta<VAR, true, false>), g, b, 0, st, P, flags, Q, QX, lam, keys, cs, ce, Qo, QXo, DP, VS);
+  else if (collide)
+    hipLaunchKernelGGL((k_delta<VAR, false, true>), g, b, 0, st, P, flags, Q, QX, lam, keys, cs, ce, Qo, QXo, DP, VS);
+  else
+    hipLaunchKernelGGL((k_delta<VAR, false, false>), g, b, 0, st, P, flags, Q, QX, lam, keys, cs, ce, Qo, QXo, DP,
+                       VS);
+}
+
+hipError_t launch_delta(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, uint32_t flags,
+                        bool delta, bool collide, const float4* Q, const float4* QX, float4* Qo, float4* QXo,
+                        const uint32_t* keys_s) {
+  dim3 g(blocks_for(P.n, tpb)), b(tpb);
+  if (P.variant == PBD_KERNEL_TEXTBOOK)
+    delta_dispatch<PBD_KERNEL_TEXTBOOK>(st, g, b, P, flags, delta, collide, Q, QX, B.LAM, keys_s, B.cstart, B.cend, Qo,
+                                        QXo, B.DP, B.VS);
+  else
+    delta_dispatch<PBD_KERNEL_REFERENCE>(st, g, b, P, flags, delta, collide, Q, QX, B.LAM, keys_s, B.cstart, B.cend,
+                                         Qo, QXo, B.DP, B.VS);
+  return hipGetLastError();
+}
+
+hipError_t launch_update(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, const float4* Q,
+                         float4* QX) {
+  hipLaunchKernelGGL(k_update, dim3(blocks_for(P.n, tpb)), dim3(tpb), 0, st, P, Q, QX, B.VS);
+  return hipGetLastError();
+}
+
+hipError_t launch_omega(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, const float4* Q,
+                        const uint32_t* keys_s) {
+  dim3 g(blocks_for(P.n, tpb)), b(tpb);
+  if (P.variant == PBD_KERNEL_TEXTBOOK)
+    hipLaunchKernelGGL(k_omega<PBD_KERNEL_TEXTBOOK>, g, b, 0, st, P, Q, B.VS, keys_s, B.cstart, B.cend, B.OM);
+  else
+    hipLaunchKernelGGL(k_omega<PBD_KERNEL_REFERENCE>, g, b, 0, st, P, Q, B.VS, keys_s, B.cstart, B.cend, B.OM);
+  return hipGetLastError();
+}
+
+hipError_t launch_final(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, uint32_t flags,
+                        const float4* Q, const float4* QX, const uint32_t* keys_s) {
+  dim3 g(blocks_for(P.n, tpb)), b(tpb);
+  if (P.variant == PBD_KERNEL_TEXTBOOK)
+    hipLaunchKernelGGL(k_final<PBD_KERNEL_TEXTBOOK>, g, b, 0, st, P, flags, Q, QX, B.VS, B.OM, keys_s, B.cstart,
+                       B.cend, B.CS, B.NCNT, B.X, B.P, B.V, B.C, B.F);
+  else
+    hipLaunchKernelGGL(k_final<PBD_KERNEL_REFERENCE>, g, b, 0, st, P, flags, Q, QX, B.VS, B.OM, keys_s, B.cstart,
+                       B.cend, B.CS, B.NCNT, B.X, B.P, B.V, B.C, B.F);
+  return hipGetLastError();
+}
+
+hipError_t launch_check_finite(hipStream_t st, uint32_t n, const float4* X, const float4* V, uint32_t* bad) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_check_finite, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, X, V, bad);
+  return hipGetLastError();
+}
+
+}  // namespace pbd

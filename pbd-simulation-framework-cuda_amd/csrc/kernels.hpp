@@ -1,0 +1,88 @@
+// kernels.hpp - host launchers of the PBF step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "params.hpp"
+
+namespace pbd {
+
+// k_delta flags
+enum : uint32_t { DF_APPLY = 1u, DF_UPDATE = 2u, DF_WRITE_DP = 4u };
+// k_final flags
+enum : uint32_t { FF_VORT = 1u, FF_XSPH = 2u, FF_UPDATED = 4u };
+
+// Device buffers of one context (see kernels.hip for the layout).
+struct StepBuffers {
+  // public, particle order of the previous step on entry, sorted order on exit
+  float4* X;   // positions4
+  float4* P;   // predictedPositions4
+  float4* V;   // velocities4
+  float4* C;   // colors4
+  float4* F;   // externalForces (by slot)
+  // step-internal, sorted order
+  float4* Q[2];
+  float4* QX[2];
+  float4* VS;
+  float4* CS;
+  float4* DP;
+  float4* OM;
+  float* LAM;
+  float* DENS;
+  uint32_t* NCNT;
+  uint32_t* CCNT;
+  uint32_t* keys_in;    // unsorted cell ids
+  uint32_t* cstart;     // cell table
+  uint32_t* cend;
+  uint32_t table_size;
+};
+
+hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n);
+hipError_t launch_predict_keys(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B);
+hipError_t launch_reorder(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,
+                          const uint32_t* keys_s, const uint32_t* ids_s);
+hipError_t launch_lambda(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, const float4* Q,
+                         const uint32_t* keys_s, bool first);
+hipError_t launch_count(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,
+                        const uint32_t* keys_s);
+hipError_t launch_delta(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, uint32_t flags,
+                        bool delta, bool collide, const float4* Q, const float4* QX, float4* Qo, float4* QXo,
+                        const uint32_t* keys_s);
+hipError_t launch_update(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, const float4* Q,
+                         float4* QX);
+hipError_t launch_omega(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, const float4* Q,
+                        const uint32_t* keys_s);
+hipError_t launch_final(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, uint32_t flags,
+                        const float4* Q, const float4* QX, const uint32_t* keys_s);
+hipError_t launch_check_finite(hipStream_t st, uint32_t n, const float4* X, const float4* V, uint32_t* bad);
+
+// ---- sort (sort.hip) ----
+struct SortPlan {
+  uint32_t n_max;       // capacity in items
+  uint32_t passes;      // 8-bit digit passes (1..4)
+  uint32_t tiles_max;   // tiles for n_max
+  // device scratch
+  uint32_t* hist;       // [4][256] digit counts then exclusive offsets
+  uint32_t* tile_ctr;   // [4] tile tickets
+  uint32_t* status;     // [4][tiles_max][256] look-back words
+  uint32_t* err;        // look-back spin-limit flag
+  uint32_t* keys_alt;   // ping-pong buffers (n_max)
+  uint32_t* vals_alt;
+  uint32_t* keys_out;
+  uint32_t* vals_out;
+};
+
+constexpr uint32_t kSortThreads = 256;
+constexpr uint32_t kSortItems = 8;
+constexpr uint32_t kSortTile = kSortThreads * kSortItems;
+
+// Number of 8-bit passes covering keys < key_bound.
+uint32_t sort_passes_for(uint32_t key_bound);
+// Stable LSD sort of (keys_in[i], i) over n items; the result is left in
+// *keys_res / *vals_res (one of the plan's buffers).
+hipError_t launch_sort(hipStream_t st, const SortPlan& plan, uint32_t n, const uint32_t* keys_in,
+                       const uint32_t** keys_res, const uint32_t** vals_res);
+
+}  // namespace pbd

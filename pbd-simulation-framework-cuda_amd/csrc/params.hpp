@@ -1,0 +1,79 @@
+// params.hpp - step constants shared by the host orchestrator and the gfx950 kernels.
+//
+// DevParams is the device-side counterpart of the reference's
+// `__constant__ SimulationParameters params` (kernel/Globals.h:50-77), filled
+// from pbd_params the way initializeFrame fills it (kernel/Initialize.h:34-71).
+// It travels to every kernel as a by-value argument (kernel arguments live in
+// SGPRs on CDNA), so a parameter change needs no symbol upload.
+#pragma once
+
+#include <cstdint>
+
+#include "pbd/pbd.h"
+
+namespace pbd {
+
+// Stage gates (Application.Simulation.Active.*, Fluid.cu:21-95).
+enum GateBits : uint32_t {
+  G_APPLY_FORCES = 1u << 0,
+  G_FIND_CONTACTS = 1u << 1,
+  G_FIND_NEIGHBOURS = 1u << 2,
+  G_LAMBDA = 1u << 3,
+  G_DELTA = 1u << 4,
+  G_APPLY_DELTA = 1u << 5,
+  G_UPDATE = 1u << 6,
+  G_OMEGA = 1u << 7,
+  G_VORTICITY = 1u << 8,
+  G_VISCOSITY = 1u << 9,
+};
+
+struct DevParams {
+  uint32_t n;          // numberOfParticles
+  uint32_t cap;        // maxNeighboursPerParticle
+  int32_t kw;          // kernelWidthNeighbours
+  int32_t variant;     // PBD_KERNEL_*
+  uint32_t gates;      // GateBits
+  float dt;            // deltaT / substeps
+  float gravity;
+  float bmin[3], bmax[3];  // enclosure
+  float lo[3], hi[3];      // enclosure inset by 0.001 (Forces.h:28,32)
+  float vdamp, pdamp;      // enclosure velocity / position damping
+  float origin[3];         // grid origin
+  float cell;              // grid cell size
+  float inv_cell;          // FP32 reciprocal of the cell size
+  uint32_t dims[3];        // cells per axis
+  float diameter;          // 2 r (Initialize.h:43)
+  float hp, hp2;           // poly6 width (Viscosity.kernelWidthPoly)
+  float hs, hs2;           // spiky width (2 r * Density.kernelWidthSpiky)
+  float wc;                // poly6 coefficient
+  float gc;                // spiky-gradient coefficient (sign included)
+  float w0;                // W(0), the textbook self term (0 for the reference variant)
+  float rho0, inv_rho0, inv_rho0_sq;
+  float eps;               // Density.relaxation
+  float k_scorr;            // s_corr k (textbook: times h_spiky^2)
+  int32_t n_scorr;
+  float inv_w_dq;          // 1 / W(dq) (0 when W(dq) = 0)
+  float coll_scale;        // -0.5 (1 - stiffness) (Collision.h:223-227)
+  float vol;               // particle volume in XSPH / vorticity (textbook 1/rho0, reference 1)
+  float c_visc;
+  float e_vort;
+  float inv_dt;
+};
+
+// Host-side derivation; returns PBD_OK or PBD_ERR_INVALID with a message.
+int derive_device_params(const pbd_params& p, uint32_t n, DevParams& out, const char** why);
+
+// Morton helpers shared by host code (table sizing) and tests.
+inline uint32_t host_spread10(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+inline uint32_t host_morton(uint32_t x, uint32_t y, uint32_t z) {
+  return (host_spread10(x) << 2) | (host_spread10(y) << 1) | host_spread10(z);
+}
+
+}  // namespace pbd

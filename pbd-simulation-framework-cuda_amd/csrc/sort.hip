@@ -1,0 +1,261 @@
+// sort.hip - stable LSD radix sort of (cell id, particle index) for gfx950.
+//
+// Replaces the reference's CUB DeviceRadixSort::SortPairs over all 32 key bits
+// (kernel/SortReorder.h:39-47; 7 digit passes x {upsweep, scan, downsweep}).
+// Only the significant bits of the Morton cell id are sorted (ceil(bits/8)
+// 8-bit passes), and each pass is one "onesweep" kernel:
+//   - a tile of 2048 keys per 256-thread workgroup, tile index from an atomic
+//     ticket so a tile only ever waits on tiles that already started;
+//   - stable in-tile ranking by wave64 multisplit (8 ballots per key give the
+//     lanes with the same digit; rank = popcount below the lane);
+//   - decoupled look-back over per-tile digit counts for the global offset;
+//   - the tile is reordered by digit in LDS and written out in runs.
+// A histogram kernel computes the digit counts of all passes in one read of
+// the keys.  The first pass takes the particle index as the value (iota), so
+// the value array is never read before it exists.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace pbd {
+
+namespace {
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagPre = 2u << 30;
+constexpr uint32_t kValMask = (1u << 30) - 1u;
+constexpr uint32_t kSpinLimit = 1u << 24;
+constexpr uint32_t kWaves = kSortThreads / 64;
+}  // namespace
+
+uint32_t sort_passes_for(uint32_t key_bound) {
+  uint32_t maxk = key_bound ? key_bound - 1u : 0u;
+  uint32_t bits = 0;
+  while (maxk) {
+    ++bits;
+    maxk >>= 1;
+  }
+  uint32_t p = (bits + 7u) / 8u;
+  return p ? p : 1u;
+}
+
+__global__ void k_sort_prepare(uint32_t* __restrict__ hist, uint32_t* __restrict__ tile_ctr,
+                               uint32_t* __restrict__ status, uint32_t status_words) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = gid; i < 4u * 256u; i += stride) hist[i] = 0u;
+  if (gid < 4u) tile_ctr[gid] = 0u;
+  for (uint32_t i = gid; i < status_words; i += stride) status[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_sort_hist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t passes,
+                                                   uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[4][256];
+  for (uint32_t i = threadIdx.x; i < 4u * 256u; i += blockDim.x) (&h[0][0])[i] = 0u;
+  __syncthreads();
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t k = keys[i];
+    for (uint32_t p = 0; p < passes; ++p) atomicAdd(&h[p][(k >> (8u * p)) & 255u], 1u);
+  }
+  __syncthreads();
+  for (uint32_t p = 0; p < passes; ++p) {
+    const uint32_t c = h[p][threadIdx.x];
+    if (c) atomicAdd(&hist[p * 256u + threadIdx.x], c);
+  }
+}
+
+// Exclusive scan of each pass's 256 digit counts (one workgroup of 256).
+__global__ __launch_bounds__(256) void k_sort_scan(uint32_t* __restrict__ hist, uint32_t passes) {
+  __shared__ uint32_t s[256];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t p = 0; p < passes; ++p) {
+    const uint32_t v = hist[p * 256u + t];
+    s[t] = v;
+    __syncthreads();
+    for (uint32_t o = 1; o < 256u; o <<= 1) {
+      const uint32_t add = t >= o ? s[t - o] : 0u;
+      __syncthreads();
+      s[t] += add;
+      __syncthreads();
+    }
+    hist[p * 256u + t] = s[t] - v;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint32_t* __restrict__ kin,
+                                                          const uint32_t* __restrict__ vin,
+                                                          uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                          uint32_t n, uint32_t shift,
+                                                          const uint32_t* __restrict__ offs,
+                                                          uint32_t* __restrict__ tile_ctr,
+                                                          uint32_t* __restrict__ status, uint32_t* __restrict__ err,
+                                                          int iota_vals) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wcnt[kWaves][256];
+  __shared__ uint32_t s_tdig[256];
+  __shared__ uint32_t s_gbase[256];
+  __shared__ uint32_t s_wsum[kWaves];
+  __shared__ uint32_t s_keys[kSortTile];
+  __shared__ uint32_t s_vals[kSortTile];
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const uint32_t wave = tid >> 6;
+  if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+  for (uint32_t i = tid; i < kWaves * 256u; i += kSortThreads) (&s_wcnt[0][0])[i] = 0u;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t base = tile * kSortTile;
+
+  uint32_t keys[kSortItems], vals[kSortItems], rank[kSortItems];
+#pragma unroll
+  for (uint32_t r = 0; r < kSortItems; ++r) {
+    const uint32_t idx = base + wave * (64u * kSortItems) + r * 64u + lane;
+    const bool valid = idx < n;
+    keys[r] = valid ? kin[idx] : 0xFFFFFFFFu;
+    vals[r] = valid ? (iota_vals ? idx : vin[idx]) : 0u;
+  }
+
+  // Stable in-wave ranking: wave w covers a contiguous range, rounds in order,
+  // lanes in order, so (wave, round, lane) is the index order.
+  const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+  for (uint32_t r = 0; r < kSortItems; ++r) {
+    const uint32_t idx = base + wave * (64u * kSortItems) + r * 64u + lane;
+    const bool valid = idx < n;
+    const uint32_t d = (keys[r] >> shift) & 255u;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (uint32_t b = 0; b < 8u; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      m &= bit ? bb : ~bb;
+    }
+    uint32_t rk = 0;
+    if (valid) {
+      const uint32_t pre = s_wcnt[wave][d];
+      rk = pre + (uint32_t)__popcll(m & lt);
+      if ((m & lt) == 0ull) s_wcnt[wave][d] = pre + (uint32_t)__popcll(m);
+    }
+    rank[r] = rk;
+  }
+  __syncthreads();
+
+  // Per digit (thread = digit): tile count, wave-exclusive offsets.
+  const uint32_t dg = tid;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kWaves; ++w) {
+    const uint32_t c = s_wcnt[w][dg];
+    s_wcnt[w][dg] = cnt;
+    cnt += c;
+  }
+  uint32_t* my_status = status + (size_t)tile * 256u + dg;
+  __hip_atomic_store(my_status, (tile == 0 ? kFlagPre : kFlagAgg) | cnt, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+
+  // Exclusive scan of the tile's digit counts.
+  uint32_t x = cnt;
+#pragma unroll
+  for (uint32_t o = 1; o < 64u; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63u) s_wsum[wave] = x;
+
+  // Decoupled look-back for this digit.
+  uint32_t excl = 0;
+  if (tile > 0) {
+    int t = (int)tile - 1;
+    uint32_t spins = 0;
+    while (t >= 0) {
+      const uint32_t v = __hip_atomic_load(status + (size_t)t * 256u + dg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t fl = v >> 30;
+      if (fl == 0u) {
+        if (++spins > kSpinLimit) {
+          atomicOr(err, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      excl += v & kValMask;
+      if (fl == 2u) break;
+      --t;
+    }
+    __hip_atomic_store(my_status, kFlagPre | ((excl + cnt) & kValMask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  s_gbase[dg] = offs[dg] + excl;
+  __syncthreads();
+  uint32_t wpre = 0;
+  for (uint32_t w = 0; w < wave; ++w) wpre += s_wsum[w];
+  s_tdig[dg] = x - cnt + wpre;
+  __syncthreads();
+
+  // Stage the tile in digit order.
+#pragma unroll
+  for (uint32_t r = 0; r < kSortItems; ++r) {
+    const uint32_t idx = base + wave * (64u * kSortItems) + r * 64u + lane;
+    if (idx < n) {
+      const uint32_t d = (keys[r] >> shift) & 255u;
+      const uint32_t pos = s_tdig[d] + s_wcnt[wave][d] + rank[r];
+      s_keys[pos] = keys[r];
+      s_vals[pos] = vals[r];
+    }
+  }
+  __syncthreads();
+  const uint32_t tile_n = (n - base) < kSortTile ? (n - base) : kSortTile;
+  for (uint32_t i = tid; i < tile_n; i += kSortThreads) {
+    const uint32_t k = s_keys[i];
+    const uint32_t d = (k >> shift) & 255u;
+    const uint32_t dst = s_gbase[d] + (i - s_tdig[d]);
+    if (dst < n) {  // always true unless a look-back hit its spin limit (reported through err)
+      kout[dst] = k;
+      vout[dst] = s_vals[i];
+    }
+  }
+}
+
+hipError_t launch_sort(hipStream_t st, const SortPlan& plan, uint32_t n, const uint32_t* keys_in,
+                       const uint32_t** keys_res, const uint32_t** vals_res) {
+  if (n == 0) {
+    *keys_res = plan.keys_out;
+    *vals_res = plan.vals_out;
+    return hipSuccess;
+  }
+  if (n > plan.n_max) return hipErrorInvalidValue;
+  const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+  const uint32_t passes = plan.passes;
+  const uint32_t status_words = passes * tiles * 256u;
+  {
+    uint32_t b = (status_words + 255u) / 256u;
+    if (b < 4u) b = 4u;
+    if (b > 2048u) b = 2048u;
+    hipLaunchKernelGGL(k_sort_prepare, dim3(b), dim3(256), 0, st, plan.hist, plan.tile_ctr, plan.status,
+                       status_words);
+  }
+  {
+    uint32_t b = (n + 256u * 16u - 1u) / (256u * 16u);
+    if (b > 1024u) b = 1024u;
+    hipLaunchKernelGGL(k_sort_hist, dim3(b), dim3(256), 0, st, keys_in, n, passes, plan.hist);
+  }
+  hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(256), 0, st, plan.hist, passes);
+  const uint32_t* kin = keys_in;
+  const uint32_t* vin = nullptr;
+  for (uint32_t p = 0; p < passes; ++p) {
+    const bool to_out = ((passes - 1u - p) % 2u) == 0u;
+    uint32_t* ko = to_out ? plan.keys_out : plan.keys_alt;
+    uint32_t* vo = to_out ? plan.vals_out : plan.vals_alt;
+    hipLaunchKernelGGL(k_onesweep, dim3(tiles), dim3(kSortThreads), 0, st, kin, vin, ko, vo, n, 8u * p,
+                       plan.hist + 256u * p, plan.tile_ctr + p, plan.status + (size_t)p * tiles * 256u, plan.err,
+                       p == 0 ? 1 : 0);
+    kin = ko;
+    vin = vo;
+  }
+  *keys_res = plan.keys_out;
+  *vals_res = plan.vals_out;
+  return hipGetLastError();
+}
+
+}  // namespace pbd

@@ -1,0 +1,379 @@
+"""ctypes binding of the C-ABI in include/pbd/pbd.h (libpbd.so).
+
+The library holds the gfx950 kernels and the C++ host code; this module only
+marshals arguments.  Loading fails loudly when libpbd.so has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libpbd.so")
+
+# status codes
+PBD_OK = 0
+PBD_ERR_INVALID = 1
+PBD_ERR_CAPACITY = 2
+PBD_ERR_HIP = 3
+PBD_ERR_CONFIG = 4
+PBD_ERR_NONFINITE = 5
+PBD_ERR_UNSUPPORTED = 6
+PBD_ERR_NOMEM = 7
+
+PBD_KERNEL_REFERENCE = 0
+PBD_KERNEL_TEXTBOOK = 1
+
+# buffer selectors
+BUF = dict(
+    positions=0,
+    predicted=1,
+    velocities=2,
+    colors=3,
+    external_forces=4,
+    densities=5,
+    lambdas=6,
+    cell_ids=7,
+    particle_ids=8,
+    cell_starts=9,
+    cell_endings=10,
+    neighbour_counters=11,
+    contact_counters=12,
+    omegas=13,
+    delta_positions=14,
+)
+_BUF_DTYPE = {
+    0: (np.float32, 4), 1: (np.float32, 4), 2: (np.float32, 4), 3: (np.float32, 4), 4: (np.float32, 4),
+    5: (np.float32, 1), 6: (np.float32, 1), 7: (np.uint32, 1), 8: (np.uint32, 1), 9: (np.uint32, 1),
+    10: (np.uint32, 1), 11: (np.uint32, 1), 12: (np.uint32, 1), 13: (np.float32, 4), 14: (np.float32, 4),
+}
+
+_f3 = C.c_float * 3
+_u3 = C.c_uint32 * 3
+
+
+class PbdParams(C.Structure):
+    """Mirror of `pbd_params` (all members are 4-byte scalars or arrays)."""
+
+    _fields_ = [
+        ("threads_per_block", C.c_uint32),
+        ("texture_width", C.c_uint32),
+        ("active_apply_forces", C.c_int32),
+        ("active_find_contacts", C.c_int32),
+        ("active_find_neighbours", C.c_int32),
+        ("active_compute_lambda", C.c_int32),
+        ("active_compute_delta_positions", C.c_int32),
+        ("active_apply_delta_positions", C.c_int32),
+        ("active_update_positions", C.c_int32),
+        ("active_compute_omegas", C.c_int32),
+        ("active_compute_vorticity", C.c_int32),
+        ("active_compute_viscosity", C.c_int32),
+        ("collision_type", C.c_int32),
+        ("kernel_width_neighbours", C.c_uint32),
+        ("max_collision_batches", C.c_uint32),
+        ("stiffness", C.c_float),
+        ("k_scorr", C.c_float),
+        ("kernel_width_spiky", C.c_float),
+        ("n_scorr", C.c_int32),
+        ("q_scorr", C.c_float),
+        ("rest_density", C.c_float),
+        ("relaxation", C.c_float),
+        ("rest_density_from_lattice", C.c_int32),
+        ("kernel_variant", C.c_int32),
+        ("enclosure_position_damping", C.c_float),
+        ("enclosure_velocity_damping", C.c_float),
+        ("bounds_min", _f3),
+        ("bounds_max", _f3),
+        ("delta_t", C.c_float),
+        ("gravity", C.c_float),
+        ("substeps", C.c_uint32),
+        ("solver_iterations", C.c_uint32),
+        ("stabilization_iterations", C.c_uint32),
+        ("max_neighbours_per_particle", C.c_uint32),
+        ("particle_radius", C.c_float),
+        ("box_width", C.c_uint32),
+        ("box_height", C.c_uint32),
+        ("box_depth", C.c_uint32),
+        ("box_spacing", C.c_float),
+        ("box_spawn_in_the_beginning", C.c_int32),
+        ("box_origin", _f3),
+        ("box_jitter", C.c_float),
+        ("box_seed_lo", C.c_uint32),
+        ("box_seed_hi", C.c_uint32),
+        ("c_viscosity", C.c_float),
+        ("kernel_width_poly", C.c_float),
+        ("e_vorticity", C.c_float),
+        ("grid_cell_size", C.c_float),
+        ("grid_origin", _f3),
+        ("grid_dims", _u3),
+    ]
+
+    def copy(self) -> "PbdParams":
+        out = PbdParams()
+        C.memmove(C.byref(out), C.byref(self), C.sizeof(PbdParams))
+        return out
+
+
+class PbdBuffers(C.Structure):
+    _fields_ = [
+        ("positions4", C.c_void_p),
+        ("predicted_positions4", C.c_void_p),
+        ("velocities4", C.c_void_p),
+        ("colors4", C.c_void_p),
+        ("external_forces4", C.c_void_p),
+        ("num_particles", C.c_uint32),
+        ("capacity", C.c_uint32),
+    ]
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "pbd_params_size": (C.c_size_t, []),
+    "pbd_params_default": (C.c_int, [C.POINTER(PbdParams)]),
+    "pbd_params_load": (C.c_int, [C.c_char_p, C.POINTER(PbdParams), C.c_char_p, C.c_size_t]),
+    "pbd_params_merge": (C.c_int, [C.c_char_p, C.POINTER(PbdParams), C.c_char_p, C.c_size_t]),
+    "pbd_params_set": (C.c_int, [C.POINTER(PbdParams), C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t]),
+    "pbd_params_get": (C.c_int, [C.POINTER(PbdParams), C.c_char_p, C.c_char_p, C.c_size_t]),
+    "pbd_params_write": (C.c_int, [C.POINTER(PbdParams), C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "pbd_generate_box": (C.c_int, [C.POINTER(PbdParams), C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                   C.POINTER(C.c_uint32)]),
+    "pbd_lattice_rest_density": (C.c_int, [C.POINTER(PbdParams), C.POINTER(C.c_double)]),
+    "pbd_morton_encode": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    "pbd_cell_key": (C.c_int, [C.POINTER(PbdParams), C.POINTER(C.c_float), C.POINTER(C.c_uint32),
+                               C.POINTER(C.c_uint32)]),
+    "pbd_create": (C.c_int, [C.POINTER(PbdParams), C.c_uint32, C.POINTER(C.c_void_p)]),
+    "pbd_set_params": (C.c_int, [C.c_void_p, C.POINTER(PbdParams)]),
+    "pbd_get_params": (C.c_int, [C.c_void_p, C.POINTER(PbdParams)]),
+    "pbd_add_particles": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbd_add_particle": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "pbd_spawn_box": (C.c_int, [C.c_void_p]),
+    "pbd_clear_particles": (C.c_int, [C.c_void_p]),
+    "pbd_step": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "pbd_step_graph": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "pbd_compute": (C.c_int, [C.c_void_p]),
+    "pbd_sync": (C.c_int, [C.c_void_p]),
+    "pbd_num_particles": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pbd_get_buffers": (C.c_int, [C.c_void_p, C.POINTER(PbdBuffers)]),
+    "pbd_download": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
+    "pbd_upload": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
+    "pbd_buffer_bytes": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_size_t)]),
+    "pbd_check_finite": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pbd_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "pbd_get_timings": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_uint64), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "pbd_reset_timings": (C.c_int, [C.c_void_p]),
+    "pbd_last_error": (C.c_char_p, [C.c_void_p]),
+    "pbd_destroy": (None, [C.c_void_p]),
+    "pbd_version": (C.c_int, []),
+    "pbd_build_target": (C.c_char_p, []),
+}
+
+
+def lib() -> C.CDLL:
+    """Load libpbd.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libpbd.so not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        if L.pbd_params_size() != C.sizeof(PbdParams):
+            raise RuntimeError("pbd_params layout mismatch between libpbd.so and the Python binding")
+        _lib = L
+    return _lib
+
+
+class PbdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[pbd status {code}] {msg}")
+        self.code = code
+
+
+def check(rc: int, what: str = "", ctx=None) -> None:
+    if rc != PBD_OK:
+        msg = lib().pbd_last_error(ctx)
+        raise PbdError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+
+def default_params() -> PbdParams:
+    p = PbdParams()
+    check(lib().pbd_params_default(C.byref(p)), "pbd_params_default")
+    return p
+
+
+def load_params(path: str) -> PbdParams:
+    p = PbdParams()
+    err = C.create_string_buffer(512)
+    rc = lib().pbd_params_load(path.encode(), C.byref(p), err, 512)
+    if rc != PBD_OK:
+        raise PbdError(rc, err.value.decode())
+    return p
+
+
+def set_param(p: PbdParams, key: str, value) -> None:
+    err = C.create_string_buffer(512)
+    rc = lib().pbd_params_set(C.byref(p), key.encode(), str(value).encode(), err, 512)
+    if rc != PBD_OK:
+        raise PbdError(rc, err.value.decode())
+
+
+def get_param(p: PbdParams, key: str) -> str:
+    buf = C.create_string_buffer(256)
+    check(lib().pbd_params_get(C.byref(p), key.encode(), buf, 256), f"pbd_params_get({key})")
+    return buf.value.decode()
+
+
+def write_params(p: PbdParams) -> str:
+    need = C.c_size_t(0)
+    check(lib().pbd_params_write(C.byref(p), None, 0, C.byref(need)), "pbd_params_write")
+    buf = C.create_string_buffer(need.value)
+    check(lib().pbd_params_write(C.byref(p), buf, need.value, C.byref(need)), "pbd_params_write")
+    return buf.value.decode()
+
+
+def generate_box(p: PbdParams):
+    """Spawn lattice of Particles.Box as (pos4, vel4, col4) float32 arrays."""
+    n = p.box_width * p.box_height * p.box_depth
+    pos = np.zeros((n, 4), np.float32)
+    vel = np.zeros((n, 4), np.float32)
+    col = np.zeros((n, 4), np.float32)
+    got = C.c_uint32(0)
+    check(lib().pbd_generate_box(C.byref(p), pos.ctypes.data, vel.ctypes.data, col.ctypes.data, n, C.byref(got)),
+          "pbd_generate_box")
+    return pos, vel, col
+
+
+def lattice_rest_density(p: PbdParams) -> float:
+    out = C.c_double(0)
+    check(lib().pbd_lattice_rest_density(C.byref(p), C.byref(out)), "pbd_lattice_rest_density")
+    return out.value
+
+
+def cell_key(p: PbdParams, pos) -> tuple[int, tuple[int, int, int]]:
+    arr = (C.c_float * 3)(*[float(v) for v in pos[:3]])
+    key = C.c_uint32(0)
+    ijk = (C.c_uint32 * 3)()
+    check(lib().pbd_cell_key(C.byref(p), arr, C.byref(key), ijk), "pbd_cell_key")
+    return key.value, (ijk[0], ijk[1], ijk[2])
+
+
+class Context:
+    """One simulation context (a Fluid on the current HIP device)."""
+
+    def __init__(self, params: PbdParams, capacity: int = 0):
+        self._ptr = C.c_void_p(None)
+        check(lib().pbd_create(C.byref(params), capacity, C.byref(self._ptr)), "pbd_create")
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def close(self) -> None:
+        if self._ptr:
+            lib().pbd_destroy(self._ptr)
+            self._ptr = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != PBD_OK:
+            msg = lib().pbd_last_error(self._ptr)
+            raise PbdError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def set_params(self, p: PbdParams) -> None:
+        self._check(lib().pbd_set_params(self._ptr, C.byref(p)), "pbd_set_params")
+
+    def params(self) -> PbdParams:
+        p = PbdParams()
+        self._check(lib().pbd_get_params(self._ptr, C.byref(p)), "pbd_get_params")
+        return p
+
+    def add_particles(self, pos4: np.ndarray, vel4: np.ndarray | None = None, col4: np.ndarray | None = None):
+        pos4 = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
+        n = pos4.shape[0]
+        v = None if vel4 is None else np.ascontiguousarray(vel4, np.float32).reshape(-1, 4)
+        c = None if col4 is None else np.ascontiguousarray(col4, np.float32).reshape(-1, 4)
+        rc = lib().pbd_add_particles(self._ptr, n, pos4.ctypes.data, None if v is None else v.ctypes.data,
+                                     None if c is None else c.ctypes.data)
+        self._check(rc, "pbd_add_particles")
+
+    def add_particle(self, pos, direction) -> None:
+        p = (C.c_float * 3)(*pos)
+        d = (C.c_float * 3)(*direction)
+        self._check(lib().pbd_add_particle(self._ptr, p, d), "pbd_add_particle")
+
+    def spawn_box(self) -> None:
+        self._check(lib().pbd_spawn_box(self._ptr), "pbd_spawn_box")
+
+    def clear(self) -> None:
+        self._check(lib().pbd_clear_particles(self._ptr), "pbd_clear_particles")
+
+    def step(self, frames: int = 1, graph: bool = False) -> None:
+        fn = lib().pbd_step_graph if graph else lib().pbd_step
+        self._check(fn(self._ptr, frames), "pbd_step")
+
+    def compute(self) -> None:
+        """One substep (Fluid::compute)."""
+        self._check(lib().pbd_compute(self._ptr), "pbd_compute")
+
+    def sync(self) -> None:
+        self._check(lib().pbd_sync(self._ptr), "pbd_sync")
+
+    @property
+    def n(self) -> int:
+        v = C.c_uint32(0)
+        self._check(lib().pbd_num_particles(self._ptr, C.byref(v)), "pbd_num_particles")
+        return v.value
+
+    def buffers(self) -> PbdBuffers:
+        b = PbdBuffers()
+        self._check(lib().pbd_get_buffers(self._ptr, C.byref(b)), "pbd_get_buffers")
+        return b
+
+    def download(self, which) -> np.ndarray:
+        w = BUF[which] if isinstance(which, str) else int(which)
+        nbytes = C.c_size_t(0)
+        self._check(lib().pbd_buffer_bytes(self._ptr, w, C.byref(nbytes)), "pbd_buffer_bytes")
+        dt, comp = _BUF_DTYPE[w]
+        count = nbytes.value // (np.dtype(dt).itemsize * comp)
+        out = np.zeros((count, comp) if comp > 1 else (count,), dt)
+        self._check(lib().pbd_download(self._ptr, w, out.ctypes.data, nbytes.value), "pbd_download")
+        return out
+
+    def upload(self, which, arr: np.ndarray) -> None:
+        w = BUF[which] if isinstance(which, str) else int(which)
+        a = np.ascontiguousarray(arr, np.float32)
+        self._check(lib().pbd_upload(self._ptr, w, a.ctypes.data, a.nbytes), "pbd_upload")
+
+    def check_finite(self) -> int:
+        bad = C.c_uint32(0)
+        rc = lib().pbd_check_finite(self._ptr, C.byref(bad))
+        if rc not in (PBD_OK, PBD_ERR_NONFINITE):
+            self._check(rc, "pbd_check_finite")
+        return bad.value
+
+    def set_timing(self, on: bool) -> None:
+        self._check(lib().pbd_set_timing(self._ptr, 1 if on else 0), "pbd_set_timing")
+
+    def reset_timings(self) -> None:
+        self._check(lib().pbd_reset_timings(self._ptr), "pbd_reset_timings")
+
+    def timings(self) -> dict:
+        cap = 32
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        launches = (C.c_uint64 * cap)()
+        count = C.c_uint32(0)
+        self._check(lib().pbd_get_timings(self._ptr, names, ms, launches, cap, C.byref(count)), "pbd_get_timings")
+        return {names[i].decode(): (ms[i], launches[i]) for i in range(count.value)}

@@ -1,0 +1,174 @@
+"""config.txt parameter contract: grammar, key set, defaults, setters, write-back.
+
+Host-only calls of libpbd.so (no GPU).  The grammar follows the reference's
+parser stack (src/parser/Config.h, ConfigParser.h, CommentParser.h,
+DefineParser.h, IncludeParser.h); the values of configs/reference.txt are the
+reference's shipped config.txt (PBD-Simulation-Framework-CUDA/config.txt:1-133).
+"""
+import ctypes
+import os
+
+import pytest
+
+from conftest import ROOT
+
+REF_DIR = "/root/reference/PBD-Simulation-Framework-CUDA"
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def fields(p):
+    out = {}
+    for name, _ in p._fields_:
+        v = getattr(p, name)
+        out[name] = tuple(v) if hasattr(v, "__len__") else v
+    return out
+
+
+def test_defaults_are_reference_config(F):
+    d = fields(F.default_params())
+    # spot values of config.txt
+    assert d["threads_per_block"] == 512 and d["texture_width"] == 256
+    assert d["kernel_width_neighbours"] == 2 and d["max_collision_batches"] == 16
+    assert d["stiffness"] == pytest.approx(0.3) and d["rest_density"] == 1200.0
+    assert d["kernel_width_spiky"] == pytest.approx(6.1) and d["n_scorr"] == 4
+    assert d["bounds_min"] == pytest.approx((1.5,) * 3) and d["bounds_max"] == pytest.approx((62.5,) * 3)
+    assert d["delta_t"] == pytest.approx(0.01) and d["gravity"] == pytest.approx(-29.82)
+    assert d["particle_radius"] == 0.5 and d["box_spacing"] == pytest.approx(1.2)
+    assert d["kernel_width_poly"] == pytest.approx(4.1) and d["c_viscosity"] == pytest.approx(1e-4)
+    assert d["relaxation"] == pytest.approx(1e-4)  # hard-coded epsilon of Density.h:90
+    assert d["grid_dims"] == (128, 128, 128)  # maxGrid = 128^3 (Particles.cc:8)
+    assert d["box_origin"] == (30.0, 32.0, 30.0)  # Particles.cc:197,205
+
+
+def test_shipped_reference_config_equals_defaults(F):
+    p = F.load_params(F.config_path("reference"))
+    d = F.default_params()
+    fp, fd = fields(p), fields(d)
+    fd["grid_cell_size"] = 1.0  # the shipped file states the unit cell explicitly
+    assert fp == fd
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_DIR), reason="reference checkout not present")
+def test_reference_config_txt_parses_to_defaults(F):
+    p = F.load_params(os.path.join(REF_DIR, "config.txt"))
+    assert fields(p) == fields(F.default_params())
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_DIR), reason="reference checkout not present")
+def test_reference_config_copy_lacks_keys(F):
+    # 'config - Copy.txt' has no Particles.Box / Vorticity scope: the reference's
+    # Config::getValue throws for them (ConfigParser.h:160-171); so do we.
+    with pytest.raises(F.PbdError) as ei:
+        F.load_params(os.path.join(REF_DIR, "config - Copy.txt"))
+    assert "Failed to retrive variable" in str(ei.value)
+
+
+def test_grammar_fixture(F):
+    p = F.default_params()
+    err = ctypes.create_string_buffer(256)
+    rc = F.capi.lib().pbd_params_merge(os.path.join(GOLDEN, "grammar_fixture.txt").encode(), ctypes.byref(p), err,
+                                       256)
+    assert rc == 0, err.value
+    assert p.gravity == pytest.approx(-12.5)  # #define substitution
+    assert p.delta_t == pytest.approx(0.02)  # trailing f
+    assert p.active_apply_forces == 1 and p.active_compute_viscosity == 0  # true / false
+    assert p.kernel_width_neighbours == 4  # define before #undef
+    # DefineParser.h:121-136 substitutes before it looks for directives, so
+    # '#undef WIDE' / '#define WIDE 9' become '#undef 4' / '#define 4 9' and
+    # WIDE keeps its first value, as in the reference.
+    assert p.box_width == 4
+    assert tuple(p.box_origin) == (1.5, 2.5, 3.5)  # array
+    assert tuple(p.grid_dims) == (64, 32, 16)  # array with blanks
+    assert p.threads_per_block == 256
+
+
+def test_missing_file_and_malformed(F, tmp_path):
+    with pytest.raises(F.PbdError) as ei:
+        F.load_params(str(tmp_path / "nope.txt"))
+    assert ei.value.code == 4
+    bad = tmp_path / "bad.txt"
+    bad.write_text("Application:\n{\n  Cuda:\n  {\n    threadsPerBlock = ;\n  }\n}\n")
+    with pytest.raises(F.PbdError) as ei:
+        F.load_params(str(bad))
+    assert "Uninitialized value" in str(ei.value)
+    bad.write_text("Application:\n{\n = 3;\n}\n")
+    with pytest.raises(F.PbdError) as ei:
+        F.load_params(str(bad))
+    assert "Invalid config file" in str(ei.value)
+
+
+def test_write_back_round_trip(F, tmp_path):
+    for name in ("reference", "dam_break_32k", "dam_break_1m"):
+        p = F.load_params(F.config_path(name))
+        text = F.write_params(p)
+        f = tmp_path / f"{name}.txt"
+        f.write_text(text)
+        assert fields(F.load_params(str(f))) == fields(p)
+    # ConfigObject::write orders defines by name and indents scopes by two blanks
+    assert "  Cuda:\n  {\n    threadsPerBlock = 512;\n  }" in F.write_params(F.default_params())
+
+
+@pytest.mark.parametrize(
+    "key,value,ok",
+    [
+        ("Application.Simulation.Forces.gravity", "-1000", True),
+        ("Application.Simulation.Forces.gravity", "1000.5", False),
+        ("Application.Simulation.Collision.kernelWidthNeighbours", "0", False),
+        ("Application.Simulation.Collision.kernelWidthNeighbours", "50", True),
+        ("Application.Simulation.Viscosity.cViscosity", "-0.1", False),
+        ("Application.Simulation.Density.restDensity", "0", False),
+        ("Application.Simulation.Density.kernelWidthSpiky", "1000", True),
+        ("Application.Simulation.Iterations.solverIterations", "101", False),
+        ("Application.Simulation.Forces.deltaT", "0", False),
+        ("Application.Simulation.Collision.collisionType", "2", False),
+        ("Application.Simulation.Collision.maxCollisionBatches", "513", False),
+        ("Application.Simulation.Collision.stiffness", "1.01", False),
+        ("Application.Simulation.Density.nSCorr", "-100", True),
+        ("Application.Simulation.Enclosure.enclosureVelocityDamping", "1.0", True),
+        ("Application.Simulation.Enclosure.X.min", "1.0", False),  # [1.5, 126.5] on the 128 unit-cell grid
+        ("Application.Simulation.Enclosure.X.max", "126.5", True),
+        ("Application.Simulation.Enclosure.Y.max", "127", False),
+        ("Application.Simulation.Density.kernelVariant", "textbook", True),
+        ("Application.Simulation.Density.kernelVariant", "sph", False),
+        ("Application.Simulation.No.such", "1", False),
+    ],
+)
+def test_setter_ranges(F, key, value, ok):
+    # console setters of kernel/Communication.h:440-624
+    p = F.default_params()
+    if ok:
+        F.set_param(p, key, value)
+    else:
+        with pytest.raises(F.PbdError):
+            F.set_param(p, key, value)
+
+
+def test_get_values(F):
+    p = F.default_params()
+    assert F.get_param(p, "Application.Simulation.Forces.gravity") == "-29.8199997"
+    assert F.get_param(p, "Application.Simulation.Grid.dims") == "[128,128,128]"
+    assert F.get_param(p, "Application.Simulation.Density.kernelVariant") == '"reference"'
+
+
+def test_dam_break_configs(F):
+    p = F.load_params(F.config_path("dam_break_32k"))
+    assert (p.box_width, p.box_height, p.box_depth) == (32, 32, 32)
+    assert p.kernel_variant == F.PBD_KERNEL_TEXTBOOK and p.rest_density_from_lattice == 1
+    assert p.substeps == 2 and p.solver_iterations == 4 and p.relaxation == 600.0
+    assert tuple(p.grid_dims) == (32, 16, 16) and p.grid_cell_size == pytest.approx(0.1)
+    assert 2 * p.particle_radius * p.kernel_width_spiky == pytest.approx(0.1)
+    p1 = F.load_params(F.config_path("dam_break_1m"))
+    assert p1.box_width * p1.box_height * p1.box_depth == 1 << 20
+    assert p1.texture_width ** 2 >= 1 << 20 and tuple(p1.grid_dims) == (128, 64, 64)
+    p2 = F.load_params(F.config_path("dam_break_2m"))
+    assert p2.box_width * p2.box_height * p2.box_depth == 128 ** 3 and p2.texture_width ** 2 >= 128 ** 3
+
+
+def test_python_config_mirror(F, tmp_path):
+    c = F.Config(F.config_path("reference"))
+    assert c.getValue("Application.Simulation.Iterations.solverIterations", int) == 1
+    c.setValue(3, "Application.Simulation.Iterations.solverIterations")
+    assert c.params.solver_iterations == 3
+    out = tmp_path / "w.txt"
+    c.write(str(out))
+    assert F.Config(str(out)).params.solver_iterations == 3
