@@ -116,7 +116,8 @@ def main():
     graph_value = None
     if not args.no_graph:
         q = p.copy()
-        q.substeps = 1  # one graph replay = one substep
+        q.delta_t = p.delta_t / p.substeps  # one graph replay = one substep of the same dt
+        q.substeps = 1
         ctx.set_params(q)
         ctx.step(2, graph=True)
         ctx.sync()
