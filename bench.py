@@ -102,6 +102,7 @@ def main():
     # timed region: K substeps, per-kernel HIP events on the context's stream
     ctx.set_timing(True)
     ctx.reset_timings()
+    ctx.reset_counters()
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -109,6 +110,7 @@ def main():
     ctx.sync()
     elapsed = time.perf_counter() - t0
     timings = ctx.timings()
+    counters = ctx.counters()
     ctx.set_timing(False)
     value = n * args.steps / elapsed
 
@@ -172,6 +174,7 @@ def main():
         "steps_per_s": args.steps / elapsed,
         "stages": stages,
         "nonfinite_particles": bad,
+        "neighbour_lists": counters,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, args.cpu_steps)

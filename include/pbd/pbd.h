@@ -124,6 +124,9 @@ typedef struct pbd_params {
   float grid_cell_size;                /* <= 0 selects the particle diameter (reference: 1.0 = 2r) */
   float grid_origin[3];
   uint32_t grid_dims[3];               /* cells per axis, each in [1, 1024]; reference 128^3 */
+  /* Application.Cuda.tileBits (new): neighbour passes on LDS tiles of 2^tileBits
+   * Morton-aligned cells (5 = 2x4x4, 6 = 4x4x4) or per particle (0) */
+  uint32_t tile_bits;
 } pbd_params;
 
 /* Device pointers of one context (valid until the next pbd_step / add / clear). */
@@ -244,6 +247,27 @@ int pbd_set_timing(pbd_ctx* ctx, int enabled);
 int pbd_get_timings(pbd_ctx* ctx, const char** names, double* ms, uint64_t* launches,
                     uint32_t cap, uint32_t* count);
 int pbd_reset_timings(pbd_ctx* ctx);
+/* Cumulative work counters since creation or the last reset (diagnostics of
+ * the neighbour passes; no counterpart in the reference):
+ *   PBD_COUNTER_LIST_BUILDS    neighbour-list builds (lambda and omega passes)
+ *   PBD_COUNTER_UNLISTED_LONG  particle-builds with more pairs in range than a
+ *                              list holds
+ *   PBD_COUNTER_UNLISTED_CAP   particle-builds whose 27-cell count exceeds
+ *                              maxNeighboursPerParticle (the truncated order of
+ *                              the reference applies: scanned per particle)
+ *   PBD_COUNTER_UNLISTED_BLOCK particle-builds in cell blocks whose halo does
+ *                              not fit in LDS (scanned per particle)
+ * An unlisted particle is scanned again by the pass that would read its list.
+ * Writes min(cap, PBD_COUNTER_COUNT) values; *count = PBD_COUNTER_COUNT. */
+enum {
+  PBD_COUNTER_LIST_BUILDS = 0,
+  PBD_COUNTER_UNLISTED_LONG = 1,
+  PBD_COUNTER_UNLISTED_CAP = 2,
+  PBD_COUNTER_UNLISTED_BLOCK = 3,
+  PBD_COUNTER_COUNT = 4
+};
+int pbd_get_counters(pbd_ctx* ctx, uint64_t* values, uint32_t cap, uint32_t* count);
+int pbd_reset_counters(pbd_ctx* ctx);
 /* Last error message of this context (or of the last failed pbd_create when ctx is NULL). */
 const char* pbd_last_error(pbd_ctx* ctx);
 /* Free the context (cudaCleanupKernels, kernel/Initialize.h:136-140). */

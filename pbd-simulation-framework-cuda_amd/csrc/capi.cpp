@@ -207,6 +207,16 @@ int pbd_reset_timings(pbd_ctx* ctx) {
   return PBD_OK;
 }
 
+int pbd_get_counters(pbd_ctx* ctx, uint64_t* values, uint32_t cap, uint32_t* count) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, { return ctx->fluid.counters(values, cap, count); })
+}
+
+int pbd_reset_counters(pbd_ctx* ctx) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, { return ctx->fluid.reset_counters(); })
+}
+
 const char* pbd_last_error(pbd_ctx* ctx) {
   if (!ctx) return g_create_error.c_str();
   return ctx->fluid.error().c_str();

@@ -272,6 +272,7 @@ struct KeyDesc {
 #define OFF(f) offsetof(pbd_params, f)
 const KeyDesc kKeys[] = {
     {"Application.Cuda.threadsPerBlock", KT::U32, OFF(threads_per_block), true, true, 64, 1024, false, false},
+    {"Application.Cuda.tileBits", KT::U32, OFF(tile_bits), false, false, 0, 0, false, false},
     {"Application.OpenGL.Textures.width", KT::U32, OFF(texture_width), true, true, 1, 65535, false, false},
     {"Application.Simulation.Active.applyDeltaPositions", KT::BOOL, OFF(active_apply_delta_positions), true, false, 0, 0, false, false},
     {"Application.Simulation.Active.applyForces", KT::BOOL, OFF(active_apply_forces), true, false, 0, 0, false, false},
@@ -580,6 +581,10 @@ int derive_device_params(const pbd_params& p, uint32_t n, DevParams& o, const ch
     *why = "substeps must be >= 1";
     return PBD_ERR_INVALID;
   }
+  if (p.tile_bits != 0 && p.tile_bits != 5 && p.tile_bits != 6) {
+    *why = "tileBits must be 0, 5 or 6";
+    return PBD_ERR_INVALID;
+  }
   if (p.collision_type != 0 && p.collision_type != 1) {
     *why = "collisionType must be 0 or 1";
     return PBD_ERR_INVALID;
@@ -664,6 +669,14 @@ int derive_device_params(const pbd_params& p, uint32_t n, DevParams& o, const ch
   // Textbook XSPH and vorticity carry the SPH particle volume m/rho0 (mass 1).
   o.vol = p.kernel_variant == PBD_KERNEL_TEXTBOOK ? (float)(1.0 / rho0) : 1.0f;
   o.c_visc = p.c_viscosity;
+  {
+    // neighbour lists of the tile passes: every pair within the largest support
+    // of any pair term; the 1e-4 margin absorbs the rounding of r^2 and of
+    // sqrt(r^2) against the diameter, so the list is a superset of the pairs
+    // any term accepts.
+    const float r = std::max(std::max(hp, hs), diam);
+    o.list_r2 = r * r * 1.0001f;
+  }
   o.e_vort = p.e_vorticity;
   return PBD_OK;
 }
@@ -734,6 +747,7 @@ extern "C" int pbd_params_default(pbd_params* o) {
     o->grid_origin[a] = 0.0f;
     o->grid_dims[a] = 128;  // maxGrid = 128^3 (Particles.cc:8)
   }
+  o->tile_bits = 5;
   return PBD_OK;
 }
 

@@ -244,16 +244,24 @@ contacts_done:
 //   reference (Density.h:14-42): W ~ ((h^2 - r^2)^3)^2 on 0.001 < r < h - 0.001;
 //                                grad ~ (h - r)^2 / r * r_vec on r > 1e-4, h - r > 1e-4
 //   textbook:                    W ~ (h^2 - r^2)^3 on r < h;  grad ~ (h - r)^2 / r * r_vec on 0 < r < h
+// r and 1/r come from the hardware reciprocal square root (v_rsq_f32, 1 ulp):
+// the IEEE sqrt and divide sequences cost ~16 instructions per candidate.
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Textbook terms are branch-free: max() clamps at the supports, which gives
+// the same zero outside them as the r < h tests, and r2 is floored at 1e-30 so
+// that 1/r stays finite for coincident particles (whose r_vec is exactly zero,
+// so every g * r_vec term vanishes; (h - r)^4 is masked explicitly).
+__device__ __forceinline__ float tb_inv_r(float r2) { return rsq(fmaxf(r2, 1.0e-30f)); }
+
 template <int V>
 __device__ __forceinline__ float w_unnorm(const DevParams& P, float r2) {
   if (V == PBD_KERNEL_TEXTBOOK) {
-    if (r2 < P.hp2) {
-      const float t = P.hp2 - r2;
-      return t * t * t;
-    }
-    return 0.0f;
+    const float t = fmaxf(P.hp2 - r2, 0.0f);
+    return t * t * t;
   } else {
-    const float r = sqrtf(r2);
+    const float r = r2 > 0.0f ? r2 * rsq(r2) : 0.0f;
     if (r > 0.001f && r < (P.hp - 0.001f)) {
       const float t = P.hp2 - r2;
       const float t3 = t * t * t;
@@ -267,24 +275,33 @@ __device__ __forceinline__ float w_unnorm(const DevParams& P, float r2) {
 template <int V>
 __device__ __forceinline__ float grad_factor(const DevParams& P, float r2, float& hr4) {
   if (V == PBD_KERNEL_TEXTBOOK) {
-    if (r2 < P.hs2 && r2 > 0.0f) {
-      const float r = sqrtf(r2);
-      const float hr = P.hs - r;
-      const float hr2 = hr * hr;
-      hr4 = hr2 * hr2;
-      return hr2 / r;
-    }
+    const float ir = tb_inv_r(r2);
+    const float hr = fmaxf(P.hs - fmaxf(r2, 1.0e-30f) * ir, 0.0f);
+    const float hr2 = hr * hr;
+    hr4 = r2 > 0.0f ? hr2 * hr2 : 0.0f;
+    return hr2 * ir;
   } else {
-    const float r = sqrtf(r2);
-    if (r > 0.0001f && (P.hs - r) > 0.0001f) {
-      const float hr = P.hs - r;
-      const float hr2 = hr * hr;
-      hr4 = hr2 * hr2;
-      return hr2 / r;
+    if (r2 > 0.0f) {
+      const float ir = rsq(r2);
+      const float r = r2 * ir;
+      if (r > 0.0001f && (P.hs - r) > 0.0001f) {
+        const float hr = P.hs - r;
+        const float hr2 = hr * hr;
+        hr4 = hr2 * hr2;
+        return hr2 * ir;
+      }
     }
   }
   hr4 = 0.0f;
   return 0.0f;
+}
+
+// x^N for a compile-time N >= 1 (square-and-multiply, N known to the compiler).
+template <int N>
+__device__ __forceinline__ float ipow_c(float x) {
+  if (N == 1) return x;
+  const float h = ipow_c<(N >= 2 ? N / 2 : 1)>(x);
+  return (N & 1) ? h * h * x : h * h;
 }
 
 __device__ __forceinline__ float ipow(float x, int n) {

@@ -56,10 +56,10 @@ int Fluid::alloc_table() {
   const uint32_t ts = host_morton(params_.grid_dims[0] - 1u, params_.grid_dims[1] - 1u, params_.grid_dims[2] - 1u) + 1u;
   if (B_.cstart && ts == B_.table_size) return PBD_OK;
   if (B_.cstart) {
-    (void)hipFree(B_.cstart);
-    (void)hipFree(B_.cend);
-    allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), (void*)B_.cstart), allocations_.end());
-    allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), (void*)B_.cend), allocations_.end());
+    for (void* q : {(void*)B_.cstart, (void*)B_.cend}) {
+      (void)hipFree(q);
+      allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), q), allocations_.end());
+    }
     B_.cstart = B_.cend = nullptr;
   }
   if (dev_alloc(allocations_, &B_.cstart, ts) || dev_alloc(allocations_, &B_.cend, ts))
@@ -111,6 +111,12 @@ int Fluid::init(const pbd_params& p, uint32_t capacity) {
   rc |= dev_alloc(allocations_, &sort_.keys_out, c);
   rc |= dev_alloc(allocations_, &sort_.vals_out, c);
   rc |= dev_alloc(allocations_, &d_scratch_, 4);
+  rc |= dev_alloc(allocations_, &tile_list_, c);
+  rc |= dev_alloc(allocations_, &tile_count_, 1);
+  rc |= dev_alloc(allocations_, &B_.NL, c * 8);  // up to 64 slots (8 chunks) per particle
+  rc |= dev_alloc(allocations_, &B_.NLC, c);
+  B_.nl_cap = cap_;
+  rc |= dev_alloc(allocations_, &B_.STATS, PBD_COUNTER_COUNT);
   if (rc) return fail(PBD_ERR_NOMEM, "device allocation failed");
   if (alloc_table()) return status_;
   // initializeDensity zeroes lambdas, deltaPositions, externalForces, omegas (Density.h:321-330)
@@ -122,6 +128,8 @@ int Fluid::init(const pbd_params& p, uint32_t capacity) {
     if (e == hipSuccess) e = hipMemsetAsync(q, 0, c * 4, stream_);
   if (e == hipSuccess) e = hipMemsetAsync(sort_.err, 0, 4, stream_);
   if (e == hipSuccess) e = hipMemsetAsync(d_scratch_, 0, 16, stream_);
+  if (e == hipSuccess) e = hipMemsetAsync(tile_count_, 0, 4, stream_);
+  if (e == hipSuccess) e = hipMemsetAsync(B_.STATS, 0, PBD_COUNTER_COUNT * 8, stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
   if (hip_check(e, "initial zeroing")) return status_;
   stage_ms_.assign(ST_NUM, 0.0);
@@ -244,6 +252,23 @@ void Fluid::reset_timings() {
   std::fill(stage_launches_.begin(), stage_launches_.end(), 0);
 }
 
+int Fluid::counters(uint64_t* values, uint32_t cap, uint32_t* count) {
+  if (count) *count = PBD_COUNTER_COUNT;
+  if (!values || cap == 0) return PBD_OK;
+  unsigned long long h[PBD_COUNTER_COUNT] = {};
+  hipError_t e = hipMemcpyAsync(h, B_.STATS, sizeof(h), hipMemcpyDeviceToHost, stream_);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  if (hip_check(e, "counters")) return status_;
+  for (uint32_t i = 0; i < std::min<uint32_t>(cap, PBD_COUNTER_COUNT); ++i) values[i] = h[i];
+  return PBD_OK;
+}
+
+int Fluid::reset_counters() {
+  hipError_t e = launch_fill_u32(stream_, reinterpret_cast<uint32_t*>(B_.STATS), 0u, 2 * PBD_COUNTER_COUNT);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  return hip_check(e, "reset counters");
+}
+
 // ---------------------------------------------------------------------------
 // The step.
 #define PBD_LAUNCH(stage, call)                                   \
@@ -259,15 +284,26 @@ int Fluid::enqueue_substep() {
   const hipStream_t st = stream_;
   const uint32_t tpb = tpb_;
   const uint32_t g = P.gates;
+  // LDS tiles for the 27-cell stencil of the textbook kernels (tiles.hip)
+  const int bits = (int)params_.tile_bits;
+  const bool tiles = bits != 0 && P.variant == PBD_KERNEL_TEXTBOOK && P.kw == 1 && (g & G_FIND_CONTACTS);
+  uint32_t tgrid = 0;
+  if (tiles) {
+    const uint32_t blocks = ((B_.table_size - 1u) >> bits) + 1u;
+    tgrid = std::min(blocks, P.n);
+  }
+  const TileList TL{tile_list_, tile_count_};
   // 1-2. applyForces + initializeCellIds (Fluid.cu:21-25)
   PBD_LAUNCH(ST_PREDICT, launch_predict_keys(st, tpb, P, B_));
   // resetCellInfo (Fluid.cu:31): empty = {UINT_MAX, N}
   PBD_LAUNCH(ST_CELL_RESET, launch_fill_u32(st, B_.cstart, 0xFFFFFFFFu, B_.table_size));
   PBD_LAUNCH(ST_CELL_RESET, launch_fill_u32(st, B_.cend, P.n, B_.table_size));
+  if (tiles) PBD_LAUNCH(ST_CELL_RESET, launch_fill_u32(st, tile_count_, 0u, 1u));
   // 3. sortIds (Fluid.cu:27)
   PBD_LAUNCH(ST_SORT, launch_sort(st, sort_, P.n, B_.keys_in, &keys_sorted_, &ids_sorted_));
   // 4-5. reorderStorage + computeCellInfo (Fluid.cu:29,33)
-  PBD_LAUNCH(ST_REORDER, launch_reorder(st, tpb, P, B_, keys_sorted_, ids_sorted_));
+  PBD_LAUNCH(ST_REORDER, launch_reorder(st, tpb, P, B_, keys_sorted_, ids_sorted_, tiles ? (uint32_t)bits : 0u,
+                                        tile_list_, tile_count_));
   const uint32_t* keys = keys_sorted_;
 
   // 6-7. solver loop (Fluid.cu:49-80); contacts/neighbours are walked inside the passes
@@ -279,8 +315,14 @@ int Fluid::enqueue_substep() {
   bool counted = false, updated = false;
   for (uint32_t it = 0; it < iters; ++it) {
     const bool last_it = it + 1 == iters;
+    bool listed = false;  // the lambda pass left the neighbour lists of Q[cur]
     if (lambda_on) {
-      PBD_LAUNCH(ST_LAMBDA, launch_lambda(st, tpb, P, B_, B_.Q[cur], keys, !counted));
+      if (tiles) {
+        PBD_LAUNCH(ST_LAMBDA, launch_tile_lambda(st, bits, tgrid, P, TL, B_, B_.Q[cur], keys, !counted));
+        listed = true;
+      } else {
+        PBD_LAUNCH(ST_LAMBDA, launch_lambda(st, tpb, P, B_, B_.Q[cur], keys, !counted));
+      }
       counted = true;
     }
     // the Q-writing kernels of this iteration
@@ -308,8 +350,13 @@ int Fluid::enqueue_substep() {
         updated = true;
       }
       const int stage = ks[k].delta ? ST_DELTA : ST_COLLIDE;
-      PBD_LAUNCH(stage, launch_delta(st, tpb, P, B_, fl, ks[k].delta, ks[k].collide, B_.Q[cur], B_.QX[cur],
-                                     B_.Q[1 - cur], B_.QX[1 - cur], keys));
+      if (tiles && ks[k].delta) {
+        PBD_LAUNCH(stage, launch_tile_delta(st, bits, tgrid, P, TL, B_, fl, ks[k].collide, listed && k == 0,
+                                            B_.Q[cur], B_.QX[cur], B_.Q[1 - cur], B_.QX[1 - cur], keys));
+      } else {
+        PBD_LAUNCH(stage, launch_delta(st, tpb, P, B_, fl, ks[k].delta, ks[k].collide, B_.Q[cur], B_.QX[cur],
+                                       B_.Q[1 - cur], B_.QX[1 - cur], keys));
+      }
       cur = 1 - cur;
     }
   }
@@ -320,13 +367,23 @@ int Fluid::enqueue_substep() {
     updated = true;
   }
   // 9. computeOmega (Fluid.cu:86-88)
-  if (g & G_OMEGA) PBD_LAUNCH(ST_OMEGA, launch_omega(st, tpb, P, B_, B_.Q[cur], keys));
+  const bool omega_listed = tiles && (g & G_OMEGA);  // the omega pass leaves the lists of Q[cur]
+  if (g & G_OMEGA) {
+    if (tiles) {
+      PBD_LAUNCH(ST_OMEGA, launch_tile_omega(st, bits, tgrid, P, TL, B_, B_.Q[cur], keys));
+    } else {
+      PBD_LAUNCH(ST_OMEGA, launch_omega(st, tpb, P, B_, B_.Q[cur], keys));
+    }
+  }
   // 10-11. computeVorticity + computeViscosity (Fluid.cu:90-96) and the output
   uint32_t ff = 0;
   if (g & G_VORTICITY) ff |= FF_VORT;
   if (g & G_VISCOSITY) ff |= FF_XSPH;
   if (updated) ff |= FF_UPDATED;
-  PBD_LAUNCH(ST_FINAL, launch_final(st, tpb, P, B_, ff, B_.Q[cur], B_.QX[cur], keys));
+  if (tiles) {
+    PBD_LAUNCH(ST_FINAL, launch_tile_final(st, bits, tgrid, P, TL, B_, ff, omega_listed, B_.Q[cur], B_.QX[cur], keys));
+  } else
+    PBD_LAUNCH(ST_FINAL, launch_final(st, tpb, P, B_, ff, B_.Q[cur], B_.QX[cur], keys));
   return PBD_OK;
 }
 

@@ -58,6 +58,8 @@ class Fluid : public Cuda_Computable {
   void set_timing(bool on) { timing_ = on; }
   int timings(const char** names, double* ms, uint64_t* launches, uint32_t cap, uint32_t* count);
   void reset_timings();
+  int counters(uint64_t* values, uint32_t cap, uint32_t* count);
+  int reset_counters();
 
   int status() const { return status_; }
   const std::string& error() const { return error_; }
@@ -88,6 +90,8 @@ class Fluid : public Cuda_Computable {
   const uint32_t* keys_sorted_ = nullptr;
   const uint32_t* ids_sorted_ = nullptr;
   uint32_t* d_scratch_ = nullptr;  // small device scratch: [0] finite count, [1] unused
+  uint32_t* tile_list_ = nullptr;  // occupied Morton blocks of the step
+  uint32_t* tile_count_ = nullptr;
 
   // graph of one substep
   hipGraph_t graph_ = nullptr;

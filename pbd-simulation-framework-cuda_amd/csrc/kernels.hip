@@ -23,6 +23,7 @@
 
 #include "device.hpp"
 #include "kernels.hpp"
+#include "particle_ops.hpp"
 
 namespace pbd {
 
@@ -58,7 +59,8 @@ __global__ void k_reorder(DevParams P, const uint32_t* __restrict__ keys_s, cons
                           const float4* __restrict__ X, const float4* __restrict__ V, const float4* __restrict__ F,
                           const float4* __restrict__ Pold, const float4* __restrict__ C, float4* __restrict__ Q,
                           float4* __restrict__ QX, float4* __restrict__ VS, float4* __restrict__ CS,
-                          uint32_t* __restrict__ cstart, uint32_t* __restrict__ cend) {
+                          uint32_t* __restrict__ cstart, uint32_t* __restrict__ cend, uint32_t tile_bits,
+                          uint32_t* __restrict__ tile_list, uint32_t* __restrict__ tile_count) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.n) return;
   const uint32_t key = keys_s[s];
@@ -67,6 +69,9 @@ __global__ void k_reorder(DevParams P, const uint32_t* __restrict__ keys_s, cons
   if (key != prev) {
     cstart[key] = s;
     if (s) cend[prev] = s;
+    // first particle of an occupied Morton block of 2^tile_bits cells
+    if (tile_bits && (s == 0 || (key >> tile_bits) != (prev >> tile_bits)))
+      tile_list[atomicAdd(tile_count, 1u)] = key >> tile_bits;
   }
   uint32_t j = ids_s[s];
   if (j >= P.n) j = P.n - 1;  // ids are a permutation of [0, n); the clamp only bounds a failed sort
@@ -96,39 +101,7 @@ __global__ void k_lambda(DevParams P, const float4* __restrict__ Q, const uint32
                          float* __restrict__ dens, uint32_t* __restrict__ ncnt, uint32_t* __restrict__ ccnt) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.n) return;
-  const float4 qi = Q[s];
-  const CellFrame f = make_frame(P, keys[s]);
-  float A = 0.0f, B = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-  uint32_t nc = 0;
-  const uint32_t cnt = visit_neighbours(
-      P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0, (P.gates & G_FIND_NEIGHBOURS) != 0, cs, ce, nc,
-      [&](uint32_t j, float bx, float by, float bz, bool) {
-        const float4 qj = Q[j];
-        const float rx = bx - qj.x, ry = by - qj.y, rz = bz - qj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        A += w_unnorm<VAR>(P, r2);
-        float hr4;
-        const float g = grad_factor<VAR>(P, r2, hr4);
-        gx += g * rx;
-        gy += g * ry;
-        gz += g * rz;
-        B += hr4;
-      });
-  const float W = P.wc * A + P.w0;
-  const float G2 = P.gc * P.gc * (gx * gx + gy * gy + gz * gz);
-  const float B2 = P.gc * P.gc * B;
-  float l;
-  if (VAR == PBD_KERNEL_TEXTBOOK) {
-    l = -(W * P.inv_rho0 - 1.0f) / ((G2 + B2) * P.inv_rho0_sq + P.eps);
-  } else {
-    l = -(W * P.inv_rho0 - 1.0f) / (B2 * P.inv_rho0 + G2 * P.inv_rho0 + P.eps);
-  }
-  lam[s] = l;
-  if (FIRST) {
-    dens[s] = W;
-    ncnt[s] = cnt;
-    ccnt[s] = nc;
-  }
+  lambda_particle<VAR, FIRST>(P, s, Q, keys, cs, ce, lam, dens, ncnt, ccnt);
 }
 
 // Neighbour / contact counts only (when computeLambda does not run first).
@@ -157,78 +130,7 @@ __global__ void k_delta(DevParams P, uint32_t flags, const float4* __restrict__ 
                         float4* __restrict__ QXo, float4* __restrict__ DP, float4* __restrict__ VS) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.n) return;
-  const float4 qi = Q[s];
-  const CellFrame f = make_frame(P, keys[s]);
-  float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  float px = 0.0f, py = 0.0f, pz = 0.0f;
-  if (DELTA || COLLIDE) {
-    const float li = DELTA ? lam[s] : 0.0f;
-    uint32_t nc = 0;
-    visit_neighbours(P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0,
-                     DELTA && (P.gates & G_FIND_NEIGHBOURS) != 0, cs, ce, nc,
-                     [&](uint32_t j, float bx, float by, float bz, bool contact) {
-                       const float4 qj = Q[j];
-                       const float rx = bx - qj.x, ry = by - qj.y, rz = bz - qj.z;
-                       const float r2 = rx * rx + ry * ry + rz * rz;
-                       if (DELTA) {
-                         const float w = P.wc * w_unnorm<VAR>(P, r2);
-                         const float sc = -P.k_scorr * ipow(w * P.inv_w_dq, P.n_scorr);
-                         float hr4;
-                         const float g = grad_factor<VAR>(P, r2, hr4);
-                         const float t = (li + lam[j] + sc) * g;
-                         ax += t * rx;
-                         ay += t * ry;
-                         az += t * rz;
-                       }
-                       if (COLLIDE && contact) {
-                         const float len = sqrtf(r2);
-                         const float ov = P.diameter - len;
-                         if (ov > 0.0f && len > 0.00001f) {
-                           // push along -(p_j - p_i) = r: coll_scale * ov * (p_j - p_i)/len
-                           const float t = -P.coll_scale * ov / len;
-                           px += t * rx;
-                           py += t * ry;
-                           pz += t * rz;
-                         }
-                       }
-                     });
-  }
-  float dx, dy, dz;
-  if (DELTA) {
-    const float sc = P.gc * P.inv_rho0;
-    dx = ax * sc;
-    dy = ay * sc;
-    dz = az * sc;
-    if (flags & DF_WRITE_DP) DP[s] = make_float4(dx, dy, dz, 0.0f);
-  } else {
-    const float4 d = (flags & DF_APPLY) ? DP[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    dx = d.x;
-    dy = d.y;
-    dz = d.z;
-  }
-  const float4 qxi = QX[s];
-  float nx = qi.x + px, ny = qi.y + py, nz = qi.z + pz;
-  float4 qx = make_float4(qxi.x + px, qxi.y + py, qxi.z + pz, qxi.w);
-  if (flags & DF_APPLY) {
-    if (VAR == PBD_KERNEL_REFERENCE) {
-      nx -= dx;
-      ny -= dy;
-      nz -= dz;
-    } else {
-      nx += dx;
-      ny += dy;
-      nz += dz;
-      nx = fminf(fmaxf(nx, P.lo[0] - f.cx), P.hi[0] - f.cx);
-      ny = fminf(fmaxf(ny, P.lo[1] - f.cy), P.hi[1] - f.cy);
-      nz = fminf(fmaxf(nz, P.lo[2] - f.cz), P.hi[2] - f.cz);
-    }
-  }
-  if (flags & DF_UPDATE) {
-    VS[s] = make_float4((nx - qx.x) / P.dt, (ny - qx.y) / P.dt, (nz - qx.z) / P.dt, (qi.w - qx.w) / P.dt);
-    qx = make_float4(nx, ny, nz, qi.w);
-  }
-  Qo[s] = make_float4(nx, ny, nz, qi.w);
-  QXo[s] = qx;
+  delta_particle<VAR, DELTA, COLLIDE>(P, flags, s, Q, QX, lam, keys, cs, ce, Qo, QXo, DP, VS);
 }
 
 // updatePositions alone (Update.h:11-32): v = (p* - x)/dt, x = p*.
@@ -242,34 +144,15 @@ __global__ void k_update(DevParams P, const float4* __restrict__ Q, float4* __re
 
 // ---------------------------------------------------------------------------
 // computeOmega: reference omega_i = sum (v_j - v_i) x grad(p_i - p_j);
-// textbook omega_i = (1/rho0) sum v_ij x grad_j W_ij = -(1/rho0) sum v_ij x grad_i W_ij
-// (SPH curl with the particle volume m/rho0, mass 1).
+// textbook omega_i = (1/rho0) sum v_ij x grad_j W_ij (SPH curl with the
+// particle volume m/rho0, mass 1).  w holds |omega_i|.
 template <int VAR>
 __global__ void k_omega(DevParams P, const float4* __restrict__ Q, const float4* __restrict__ VS,
                         const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
                         const uint32_t* __restrict__ ce, float4* __restrict__ OM) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.n) return;
-  const float4 qi = Q[s], vi = VS[s];
-  const CellFrame f = make_frame(P, keys[s]);
-  float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-  uint32_t nc = 0;
-  visit_neighbours(P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0, (P.gates & G_FIND_NEIGHBOURS) != 0, cs,
-                   ce, nc, [&](uint32_t j, float bx, float by, float bz, bool) {
-                     const float4 qj = Q[j];
-                     const float rx = bx - qj.x, ry = by - qj.y, rz = bz - qj.z;
-                     const float r2 = rx * rx + ry * ry + rz * rz;
-                     float hr4;
-                     const float g = grad_factor<VAR>(P, r2, hr4);
-                     const float4 vj = VS[j];
-                     const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
-                     const float gx = g * rx, gy = g * ry, gz = g * rz;
-                     ox += vy * gz - vz * gy;
-                     oy += vz * gx - vx * gz;
-                     oz += vx * gy - vy * gx;
-                   });
-  const float sc = (VAR == PBD_KERNEL_TEXTBOOK) ? -P.gc * P.vol : P.gc;
-  OM[s] = make_float4(ox * sc, oy * sc, oz * sc, 0.0f);
+  omega_particle<VAR>(P, s, Q, VS, keys, cs, ce, OM);
 }
 
 // ---------------------------------------------------------------------------
@@ -285,84 +168,7 @@ __global__ void k_final(DevParams P, uint32_t flags, const float4* __restrict__ 
                         float4* __restrict__ Vo, float4* __restrict__ Co, float4* __restrict__ Fo) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.n) return;
-  const float4 qi = Q[s], vi = VS[s];
-  const CellFrame f = make_frame(P, keys[s]);
-  const bool vort = (flags & FF_VORT) != 0, xsph = (flags & FF_XSPH) != 0;
-  float4 oi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (vort) oi = OM[s];
-  const float oil = sqrtf(oi.x * oi.x + oi.y * oi.y + oi.z * oi.z);
-  float sx = 0.0f, sy = 0.0f, sz = 0.0f, sw = 0.0f;  // XSPH sum
-  float ex = 0.0f, ey = 0.0f, ez = 0.0f;             // vorticity gradient
-  if (vort || xsph) {
-    uint32_t nc = 0;
-    visit_neighbours(P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0,
-                     (P.gates & G_FIND_NEIGHBOURS) != 0, cs, ce, nc,
-                     [&](uint32_t j, float bx, float by, float bz, bool) {
-                       const float4 qj = Q[j];
-                       const float rx = bx - qj.x, ry = by - qj.y, rz = bz - qj.z;
-                       const float r2 = rx * rx + ry * ry + rz * rz;
-                       if (xsph) {
-                         const float w = w_unnorm<VAR>(P, r2);
-                         const float4 vj = VS[j];
-                         sx += (vj.x - vi.x) * w;
-                         sy += (vj.y - vi.y) * w;
-                         sz += (vj.z - vi.z) * w;
-                         sw += (vj.w - vi.w) * w;
-                       }
-                       if (vort) {
-                         const float4 oj = OM[j];
-                         if (VAR == PBD_KERNEL_REFERENCE) {
-                           const float dx = oj.x - oi.x, dy = oj.y - oi.y, dz = oj.z - oi.z;
-                           const float ol = sqrtf(dx * dx + dy * dy + dz * dz);
-                           ex += ol / (-rx + 0.00000001f);
-                           ey += ol / (-ry + 0.00000001f);
-                           ez += ol / (-rz + 0.00000001f);
-                         } else {
-                           float hr4;
-                           const float g = grad_factor<VAR>(P, r2, hr4);
-                           const float d = sqrtf(oj.x * oj.x + oj.y * oj.y + oj.z * oj.z) - oil;
-                           ex += d * g * rx;
-                           ey += d * g * ry;
-                           ez += d * g * rz;
-                         }
-                       }
-                     });
-  }
-  if (vort) {
-    float nx, ny, nz;
-    if (VAR == PBD_KERNEL_REFERENCE) {
-      const float EPS = 0.00000001f;
-      const float inv = 1.0f / (sqrtf(ex * ex + ey * ey + ez * ez) + EPS);
-      nx = inv * (ex + EPS);
-      ny = inv * (ey + EPS);
-      nz = inv * (ez + EPS);
-    } else {
-      // eta carries the factor gc; its direction is sign(gc) * eta
-      const float sg = P.gc < 0.0f ? -1.0f : 1.0f;
-      const float el = sqrtf(ex * ex + ey * ey + ez * ez);
-      const float inv = el > 0.0f ? sg / el : 0.0f;
-      nx = ex * inv;
-      ny = ey * inv;
-      nz = ez * inv;
-    }
-    Fo[s] = make_float4(P.e_vort * (ny * oi.z - nz * oi.y), P.e_vort * (nz * oi.x - nx * oi.z),
-                        P.e_vort * (nx * oi.y - ny * oi.x), 0.0f);
-  }
-  float4 v = vi;
-  if (xsph) {
-    const float c = P.c_visc * P.wc * P.vol;
-    v = make_float4(vi.x + c * sx, vi.y + c * sy, vi.z + c * sz, vi.w + c * sw);
-  }
-  const float4 xl = (flags & FF_UPDATED) ? qi : QX[s];
-  Xo[s] = make_float4(xl.x + f.cx, xl.y + f.cy, xl.z + f.cz, xl.w);
-  Po[s] = make_float4(qi.x + f.cx, qi.y + f.cy, qi.z + f.cz, qi.w);
-  Vo[s] = v;
-  if (P.gates & G_FIND_NEIGHBOURS) {
-    const float q = (float)ncnt[s] / (float)P.cap;
-    Co[s] = make_float4(q, q, q, 1.0f);
-  } else {
-    Co[s] = CS[s];
-  }
+  final_particle<VAR>(P, flags, s, Q, QX, VS, OM, keys, cs, ce, CS, ncnt, Xo, Po, Vo, Co, Fo);
 }
 
 // Finite check of positions and velocities (failure detection).
@@ -393,9 +199,10 @@ hipError_t launch_predict_keys(hipStream_t st, uint32_t tpb, const DevParams& P,
 }
 
 hipError_t launch_reorder(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,
-                          const uint32_t* keys_s, const uint32_t* ids_s) {
+                          const uint32_t* keys_s, const uint32_t* ids_s, uint32_t tile_bits, uint32_t* tile_list,
+                          uint32_t* tile_count) {
   hipLaunchKernelGGL(k_reorder, dim3(blocks_for(P.n, tpb)), dim3(tpb), 0, st, P, keys_s, ids_s, B.X, B.V, B.F, B.P,
-                     B.C, B.Q[0], B.QX[0], B.VS, B.CS, B.cstart, B.cend);
+                     B.C, B.Q[0], B.QX[0], B.VS, B.CS, B.cstart, B.cend, tile_bits, tile_list, tile_count);
   return hipGetLastError();
 }
 

@@ -37,12 +37,24 @@ struct StepBuffers {
   uint32_t* cstart;     // cell table
   uint32_t* cend;
   uint32_t table_size;
+  // neighbour lists of the tile passes (tiles.hip)
+  uint4* NL;            // [8][nl_cap] chunks of 8 u16 LDS slots
+  uint16_t* NLC;        // padded list lengths (0xFFFF = scan instead)
+  uint32_t nl_cap;
+  unsigned long long* STATS;  // work counters (PBD_COUNTER_*)
+};
+
+// Occupied Morton blocks of the current step (built by the reorder kernel).
+struct TileList {
+  const uint32_t* list;   // block ids (cell id >> bits), unordered
+  const uint32_t* count;  // number of entries
 };
 
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n);
 hipError_t launch_predict_keys(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B);
 hipError_t launch_reorder(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,
-                          const uint32_t* keys_s, const uint32_t* ids_s);
+                          const uint32_t* keys_s, const uint32_t* ids_s, uint32_t tile_bits, uint32_t* tile_list,
+                          uint32_t* tile_count);
 hipError_t launch_lambda(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, const float4* Q,
                          const uint32_t* keys_s, bool first);
 hipError_t launch_count(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,
@@ -57,6 +69,20 @@ hipError_t launch_omega(hipStream_t st, uint32_t tpb, const DevParams& P, const 
 hipError_t launch_final(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, uint32_t flags,
                         const float4* Q, const float4* QX, const uint32_t* keys_s);
 hipError_t launch_check_finite(hipStream_t st, uint32_t n, const float4* X, const float4* V, uint32_t* bad);
+
+// ---- LDS cell-block tiles (tiles.hip), textbook variant, 27-cell stencil ----
+// The lambda and omega passes write the neighbour lists that the following
+// delta-p and final passes read when use_list is set.
+hipError_t launch_tile_lambda(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
+                              const StepBuffers& B, const float4* Q, const uint32_t* keys, bool first);
+hipError_t launch_tile_delta(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
+                             const StepBuffers& B, uint32_t flags, bool collide, bool use_list, const float4* Q,
+                             const float4* QX, float4* Qo, float4* QXo, const uint32_t* keys);
+hipError_t launch_tile_omega(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
+                             const StepBuffers& B, const float4* Q, const uint32_t* keys);
+hipError_t launch_tile_final(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
+                             const StepBuffers& B, uint32_t flags, bool use_list, const float4* Q, const float4* QX,
+                             const uint32_t* keys);
 
 // ---- sort (sort.hip) ----
 struct SortPlan {

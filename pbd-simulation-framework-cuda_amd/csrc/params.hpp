@@ -58,6 +58,7 @@ struct DevParams {
   float c_visc;
   float e_vort;
   float inv_dt;
+  float list_r2;           // neighbour-list radius^2: max(h_poly, h_spiky, diameter)^2 (padded)
 };
 
 // Host-side derivation; returns PBD_OK or PBD_ERR_INVALID with a message.

@@ -1,0 +1,655 @@
+// tiles.hip - LDS cell-block tile versions of the neighbour passes (gfx950).
+//
+// Particles are sorted by Morton cell id, so an aligned Morton block of 2^BITS
+// cells (2x4x4 cells for BITS = 5, 4x4x4 for BITS = 6) is one contiguous range
+// of the sorted arrays.  One 256-thread workgroup takes one occupied block:
+//   1. it looks up the (EX+2)(EY+2)(EZ+2) halo cells in the cell table and
+//      scans their counts into LDS offsets;
+//   2. it copies the halo particles from HBM/L2 into LDS in block-local
+//      coordinates (one float4 each, the pass's per-neighbour payload in .w:
+//      lambda for the delta-p pass, |omega| for the vorticity pass);
+//   3. every thread walks the 27-cell stencil of its own particles out of LDS.
+// The per-pair arithmetic and the per-particle epilogues are the ones of the
+// per-particle kernels (particle_ops.hpp).  A block whose halo does not fit in
+// LDS, and a particle whose 27-cell count exceeds maxNeighboursPerParticle
+// (where the reference's truncation order matters), take the per-particle
+// global-memory path.  Only the 27-cell stencil (kernelWidthNeighbours = 1,
+// findContacts on) is tiled; wider stencils use the per-particle kernels.
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "kernels.hpp"
+#include "particle_ops.hpp"
+
+namespace pbd {
+
+template <int BITS>
+struct TileGeom {
+  static constexpr int EZ = 1 << ((BITS + 2) / 3);  // Morton bit order from the LSB is z, y, x, z, ...
+  static constexpr int EY = 1 << ((BITS + 1) / 3);
+  static constexpr int EX = 1 << (BITS / 3);
+  static constexpr int HX = EX + 2, HY = EY + 2, HZ = EZ + 2;
+  static constexpr int NH = HX * HY * HZ;  // halo cells
+  static constexpr int SX = HY * HZ, SY = HZ;
+};
+
+constexpr uint32_t kTileThreads = 256;
+
+template <int BITS>
+struct TileShared {
+  uint32_t lstart[TileGeom<BITS>::NH + 1];  // LDS start of each halo cell; lstart[NH] = total
+  uint32_t gstart[TileGeom<BITS>::NH];      // sorted index of the cell's first particle
+  uint32_t wsum[kTileThreads / 64];
+  uint32_t g0, g1;                          // the block's own particle range
+};
+
+// Steps 1 of the tile: halo cell table, LDS offsets, own range.  Returns the
+// halo particle count.
+template <int BITS>
+__device__ __forceinline__ uint32_t tile_setup(const DevParams& P, uint32_t blk, const uint32_t* __restrict__ cs,
+                                               const uint32_t* __restrict__ ce, TileShared<BITS>& sh, int& ix0,
+                                               int& iy0, int& iz0) {
+  using G = TileGeom<BITS>;
+  static_assert(G::NH <= (int)kTileThreads, "one halo cell per thread");
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t first = blk << BITS;
+  ix0 = (int)compact10(first >> 2);
+  iy0 = (int)compact10(first >> 1);
+  iz0 = (int)compact10(first);
+  if (tid == 0) {
+    sh.g0 = 0xFFFFFFFFu;
+    sh.g1 = 0u;
+  }
+  __syncthreads();
+  uint32_t cnt = 0, st = 0;
+  if (tid < (uint32_t)G::NH) {
+    const int hx = (int)tid / G::SX, hy = ((int)tid / G::SY) % G::HY, hz = (int)tid % G::HZ;
+    const int cx = ix0 - 1 + hx, cy = iy0 - 1 + hy, cz = iz0 - 1 + hz;
+    if (cx >= 0 && cy >= 0 && cz >= 0 && cx < (int)P.dims[0] && cy < (int)P.dims[1] && cz < (int)P.dims[2]) {
+      const uint32_t k = morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+      const uint32_t s0 = cs[k];
+      if (s0 != 0xFFFFFFFFu) {
+        st = s0;
+        cnt = ce[k] - s0;
+        if (hx >= 1 && hx <= G::EX && hy >= 1 && hy <= G::EY && hz >= 1 && hz <= G::EZ) {
+          atomicMin(&sh.g0, st);
+          atomicMax(&sh.g1, st + cnt);
+        }
+      }
+    }
+    sh.gstart[tid] = st;
+  }
+  uint32_t x = cnt;
+#pragma unroll
+  for (uint32_t o = 1; o < 64u; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63u) sh.wsum[wave] = x;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (uint32_t w = 0; w < wave; ++w) pre += sh.wsum[w];
+  if (tid < (uint32_t)G::NH) sh.lstart[tid] = pre + x - cnt;
+  if (tid == (uint32_t)G::NH - 1u) sh.lstart[G::NH] = pre + x;
+  __syncthreads();
+  return sh.lstart[G::NH];
+}
+
+// Halo cell holding LDS slot i (largest c with lstart[c] <= i).
+template <int BITS>
+__device__ __forceinline__ int tile_cell_of(const TileShared<BITS>& sh, uint32_t i) {
+  int lo = 0, hi = TileGeom<BITS>::NH;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (sh.lstart[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Offset of halo cell c's corner from the block's first cell's corner.
+template <int BITS>
+__device__ __forceinline__ float3 tile_offset(const DevParams& P, int c, int ix0, int iy0, int iz0) {
+  using G = TileGeom<BITS>;
+  const int hx = c / G::SX, hy = (c / G::SY) % G::HY, hz = c % G::HZ;
+  return make_float3(cell_corner(P.origin[0], P.cell, (uint32_t)(ix0 - 1 + hx)) - cell_corner(P.origin[0], P.cell, (uint32_t)ix0),
+                     cell_corner(P.origin[1], P.cell, (uint32_t)(iy0 - 1 + hy)) - cell_corner(P.origin[1], P.cell, (uint32_t)iy0),
+                     cell_corner(P.origin[2], P.cell, (uint32_t)(iz0 - 1 + hz)) - cell_corner(P.origin[2], P.cell, (uint32_t)iz0));
+}
+
+// Own particle g: its halo cell and LDS slot.
+template <int BITS>
+__device__ __forceinline__ void tile_locate(const TileShared<BITS>& sh, uint32_t key, uint32_t g, int ix0, int iy0,
+                                            int iz0, int& h, uint32_t& slot) {
+  using G = TileGeom<BITS>;
+  const int hx = (int)compact10(key >> 2) - ix0 + 1;
+  const int hy = (int)compact10(key >> 1) - iy0 + 1;
+  const int hz = (int)compact10(key) - iz0 + 1;
+  h = hx * G::SX + hy * G::SY + hz;
+  slot = sh.lstart[h] + (g - sh.gstart[h]);
+}
+
+// Walk the 27 stencil cells of halo cell h; body(j) for every LDS slot j != slot.
+template <int BITS, class Body>
+__device__ __forceinline__ uint32_t tile_stencil(const TileShared<BITS>& sh, int h, uint32_t slot, Body body) {
+  using G = TileGeom<BITS>;
+  uint32_t total = 0;
+#pragma unroll
+  for (int dx = -1; dx <= 1; ++dx) {
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int hb = h + dx * G::SX + dy * G::SY;
+      // the three z cells of a (dx, dy) column are consecutive halo cells: one range
+      const uint32_t a = sh.lstart[hb - 1], b = sh.lstart[hb + 2];
+      total += b - a;
+      for (uint32_t j = a; j < b; ++j) {
+        if (j != slot) body(j);
+      }
+    }
+  }
+  return total - 1u;  // neighbour count (self excluded)
+}
+
+// 27-cell count of a particle (to decide whether the cap binds).
+template <int BITS>
+__device__ __forceinline__ uint32_t tile_count(const TileShared<BITS>& sh, int h) {
+  using G = TileGeom<BITS>;
+  uint32_t total = 0;
+#pragma unroll
+  for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int hb = h + dx * G::SX + dy * G::SY;
+      total += sh.lstart[hb + 2] - sh.lstart[hb - 1];
+    }
+  return total - 1u;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour lists.
+//
+// The lambda pass scans the 27-cell stencil of each particle with a bare
+// distance test, keeps the LDS slots of the pairs within R = max(h_poly,
+// h_spiky, diameter) (padded for rounding) in a per-thread LDS list, evaluates
+// the pass over that list, and writes it out.  The delta-p pass of the same
+// Jacobi iteration sees the same positions, the same cells and so the same LDS
+// layout: it reads the list instead of scanning.  The omega pass and the final
+// (vorticity + XSPH) pass form the same pair.  Pairs outside R contribute
+// exactly zero to every term and the list keeps the scan order, so the sums
+// are the scan's sums.
+//
+// A list is padded to a multiple of 8 entries with the sentinel slot CAP (a halo
+// entry placed far away, for which every pair term is zero) and stored as
+// chunks of 8 u16 slots, chunk-major (NL[c * cap + g]) so that a wave's loads
+// of chunk c are contiguous.  A particle with more than kNlMax pairs in range,
+// or whose 27-cell count exceeds the neighbour cap, is marked kNlInvalid: its
+// passes scan.
+constexpr uint32_t kNlMax = 64;
+constexpr uint16_t kNlInvalid = 0xFFFFu;
+constexpr float kFar = 1.0e15f;  // sentinel coordinate: r^2 ~ 3e30 stays finite
+
+struct NeighbourList {
+  uint4* nl;      // [kNlMax / 8][cap] chunks of 8 slots
+  uint16_t* cnt;  // [cap] padded entry count, or kNlInvalid
+  uint32_t cap;
+  unsigned long long* stats;  // PBD_COUNTER_LIST_*
+};
+
+// Halo particles held in LDS; the same for the producing and the consuming pass
+// so that a slot names the same particle in both.
+template <int BITS>
+struct TileCap;
+template <>
+struct TileCap<5> {
+  static constexpr int value = 1920;  // 2x4x4 blocks: 144 halo cells, ~1150 particles at rest density
+};
+template <>
+struct TileCap<6> {
+  static constexpr int value = 2880;  // 4x4x4 blocks: 216 halo cells, ~1730 at rest density
+};
+// LDS entries: the halo, the sentinel at CAP and 3 more that a scan unrolled
+// by 4 may read past a cell range (their hits are masked off).
+template <int BITS>
+constexpr int tile_lds() { return TileCap<BITS>::value + 4; }
+
+// Copy the block's halo into LDS (block-local coordinates; .w = payload(i)) and
+// place the sentinel at slot CAP.
+template <int BITS, int CAP, class Payload>
+__device__ __forceinline__ void tile_load(const DevParams& P, const TileShared<BITS>& sh, uint32_t total,
+                                          const float4* __restrict__ Q, int ix0, int iy0, int iz0, float4* s_q,
+                                          Payload payload) {
+  for (uint32_t i = threadIdx.x; i < total; i += kTileThreads) {
+    const int c = tile_cell_of<BITS>(sh, i);
+    const uint32_t gi = sh.gstart[c] + (i - sh.lstart[c]);
+    const float4 q = Q[gi];
+    const float3 o = tile_offset<BITS>(P, c, ix0, iy0, iz0);
+    s_q[i] = make_float4(q.x + o.x, q.y + o.y, q.z + o.z, payload(gi));
+  }
+  if (threadIdx.x < 4) s_q[CAP + threadIdx.x] = make_float4(kFar, kFar, kFar, 0.0f);
+}
+
+// Visit the global list of particle g (n = padded length) in order.
+template <class Body>
+__device__ __forceinline__ void nl_visit(const NeighbourList& L, uint32_t g, uint32_t n, Body body) {
+  const uint4* p = L.nl + g;
+  const uint32_t nc = n >> 3;
+  if (nc == 0) return;
+  uint4 w = p[0];
+  for (uint32_t c = 1; c <= nc; ++c) {
+    const uint4 nx = c < nc ? p[(size_t)c * L.cap] : w;
+    body(w.x & 0xFFFFu);
+    body(w.x >> 16);
+    body(w.y & 0xFFFFu);
+    body(w.y >> 16);
+    body(w.z & 0xFFFFu);
+    body(w.z >> 16);
+    body(w.w & 0xFFFFu);
+    body(w.w >> 16);
+    w = nx;
+  }
+}
+
+// In-range bits of one stencil column: bit k of (lo, hi) is set when slot a + k
+// (k < b - a <= 64) is within the list radius.  Unrolled by 4: a group may read
+// up to 3 slots past b (still inside the LDS array); their bits are masked off.
+__device__ __forceinline__ void nl_column(const float4* s_q, uint32_t a, uint32_t b, float qx, float qy, float qz,
+                                          float r2max, uint32_t& lo, uint32_t& hi) {
+  lo = 0u;
+  hi = 0u;
+  for (uint32_t k = 0; a + k < b; k += 4) {
+    float4 c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = s_q[a + k + u];
+    uint32_t g4 = 0u;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float rx = qx - c[u].x, ry = qy - c[u].y, rz = qz - c[u].z;
+      g4 |= (rx * rx + ry * ry + rz * rz < r2max) ? (1u << u) : 0u;
+    }
+    if (k < 32u) lo |= g4 << k;
+    else hi |= g4 << (k - 32u);
+  }
+  const uint32_t len = b - a;
+  if (len < 32u) lo &= (1u << len) - 1u;
+  if (len < 64u) hi &= len <= 32u ? 0u : (1u << (len - 32u)) - 1u;
+}
+
+// Appends slots to 8-entry chunks held in four registers and stores each full
+// chunk to the global list (chunk-major).
+struct ChunkWriter {
+  uint32_t w0 = 0u, w1 = 0u, w2 = 0u, w3 = 0u, k = 0u, c = 0u;
+  __device__ __forceinline__ void push(const NeighbourList& L, uint32_t g, uint32_t slot) {
+    // shift the chunk down one entry; the new slot enters at the top
+    w0 = __builtin_amdgcn_alignbit(w1, w0, 16u);
+    w1 = __builtin_amdgcn_alignbit(w2, w1, 16u);
+    w2 = __builtin_amdgcn_alignbit(w3, w2, 16u);
+    w3 = __builtin_amdgcn_alignbit(slot, w3, 16u);
+    if (++k == 8u) {
+      L.nl[(size_t)c * L.cap + g] = make_uint4(w0, w1, w2, w3);
+      ++c;
+      k = 0u;
+    }
+  }
+};
+
+__device__ __forceinline__ void nl_push_bits(ChunkWriter& cw, const NeighbourList& L, uint32_t g, uint32_t base,
+                                             uint32_t m) {
+  while (m) {
+    const uint32_t k = __builtin_ctz(m);
+    m &= m - 1u;
+    cw.push(L, g, base + k);
+  }
+}
+
+// Build the list of the particle at `slot` (halo cell h) into the global list,
+// padded to a multiple of 8 with the sentinel slot CAP.  Returns the padded
+// length, or kNlInvalid when a stencil column holds more than 64 particles or
+// more than kNlMax pairs are in range (the list is not written then).
+template <int BITS, int CAP>
+__device__ __forceinline__ uint32_t nl_build(const DevParams& P, const NeighbourList& L, uint32_t g,
+                                             const TileShared<BITS>& sh, const float4* s_q, int h, uint32_t slot,
+                                             float4 qi) {
+  using G = TileGeom<BITS>;
+  uint32_t lo[9], hi[9], base[9];
+  uint32_t n = 0;
+  bool wide = false;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    const int hb = h + (c / 3 - 1) * G::SX + (c % 3 - 1) * G::SY;
+    const uint32_t a = sh.lstart[hb - 1], b = sh.lstart[hb + 2];
+    base[c] = a;
+    wide |= b - a > 64u;
+    nl_column(s_q, a, b < a + 64u ? b : a + 64u, qi.x, qi.y, qi.z, P.list_r2, lo[c], hi[c]);
+    if (c == 4) {  // the own column: drop the particle itself
+      const uint32_t k = slot - a;
+      if (k < 32u) lo[c] &= ~(1u << k);
+      else hi[c] &= ~(1u << (k - 32u));
+    }
+    n += __builtin_popcount(lo[c]) + __builtin_popcount(hi[c]);
+  }
+  if (wide || n > kNlMax) return kNlInvalid;
+  ChunkWriter cw;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    nl_push_bits(cw, L, g, base[c], lo[c]);
+    nl_push_bits(cw, L, g, base[c] + 32u, hi[c]);
+  }
+  while (cw.k != 0u) cw.push(L, g, (uint32_t)CAP);
+  return cw.c * 8u;
+}
+
+template <int BITS, bool FIRST>
+__global__ __launch_bounds__(kTileThreads) void k_tile_lambda(DevParams P, TileList T, NeighbourList L,
+                                                              const float4* __restrict__ Q,
+                                                              const uint32_t* __restrict__ keys,
+                                                              const uint32_t* __restrict__ cs,
+                                                              const uint32_t* __restrict__ ce, float* __restrict__ lam,
+                                                              float* __restrict__ dens, uint32_t* __restrict__ ncnt,
+                                                              uint32_t* __restrict__ ccnt) {
+  constexpr int VAR = PBD_KERNEL_TEXTBOOK;
+  constexpr int CAP = TileCap<BITS>::value;
+  __shared__ TileShared<BITS> sh;
+  __shared__ float4 s_q[tile_lds<BITS>()];
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_LIST_BUILDS], 1ull);
+  if (blockIdx.x >= *T.count) return;
+  int ix0, iy0, iz0;
+  const uint32_t total = tile_setup<BITS>(P, T.list[blockIdx.x], cs, ce, sh, ix0, iy0, iz0);
+  const uint32_t g0 = sh.g0, g1 = sh.g1;
+  if (total > (uint32_t)CAP) {
+    for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
+      lambda_particle<VAR, FIRST>(P, g, Q, keys, cs, ce, lam, dens, ncnt, ccnt);
+      L.cnt[g] = kNlInvalid;
+    }
+    if (threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_BLOCK], (unsigned long long)(g1 - g0));
+    return;
+  }
+  tile_load<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, [](uint32_t) { return 0.0f; });
+  __syncthreads();
+  for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
+    int h;
+    uint32_t slot;
+    tile_locate<BITS>(sh, keys[g], g, ix0, iy0, iz0, h, slot);
+    const uint32_t cnt = tile_count<BITS>(sh, h);
+    if (cnt > P.cap) {
+      lambda_particle<VAR, FIRST>(P, g, Q, keys, cs, ce, lam, dens, ncnt, ccnt);
+      L.cnt[g] = kNlInvalid;
+      atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_CAP], 1ull);
+      continue;
+    }
+    const float4 qi = s_q[slot];
+    LambdaAcc a;
+    const auto pair = [&](uint32_t j) {
+      const float4 qj = s_q[j];
+      lambda_pair<VAR>(P, qi.x - qj.x, qi.y - qj.y, qi.z - qj.z, a);
+    };
+    const uint32_t n = nl_build<BITS, CAP>(P, L, g, sh, s_q, h, slot, qi);
+    L.cnt[g] = (uint16_t)n;
+    if (n != kNlInvalid) {
+      nl_visit(L, g, n, pair);
+    } else {
+      tile_stencil<BITS>(sh, h, slot, pair);
+      atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_LONG], 1ull);
+    }
+    float W;
+    lam[g] = lambda_finish<VAR>(P, a, W);
+    if (FIRST) {
+      dens[g] = W;
+      ncnt[g] = cnt;
+      ccnt[g] = cnt;
+    }
+  }
+}
+
+// Delta-p (+ collisions, apply, update) over the lambda pass's lists (use_list)
+// or a scan of the stencil.
+template <int BITS, bool COLLIDE, int NS>
+__global__ __launch_bounds__(kTileThreads) void k_tile_delta(DevParams P, TileList T, NeighbourList L, int use_list,
+                                                             uint32_t flags, const float4* __restrict__ Q,
+                                                             const float4* __restrict__ QX,
+                                                             const float* __restrict__ lam,
+                                                             const uint32_t* __restrict__ keys,
+                                                             const uint32_t* __restrict__ cs,
+                                                             const uint32_t* __restrict__ ce, float4* __restrict__ Qo,
+                                                             float4* __restrict__ QXo, float4* __restrict__ DP,
+                                                             float4* __restrict__ VS) {
+  constexpr int VAR = PBD_KERNEL_TEXTBOOK;
+  constexpr int CAP = TileCap<BITS>::value;
+  __shared__ TileShared<BITS> sh;
+  __shared__ float4 s_q[tile_lds<BITS>()];
+  if (blockIdx.x >= *T.count) return;
+  int ix0, iy0, iz0;
+  const uint32_t total = tile_setup<BITS>(P, T.list[blockIdx.x], cs, ce, sh, ix0, iy0, iz0);
+  const uint32_t g0 = sh.g0, g1 = sh.g1;
+  if (total > (uint32_t)CAP) {
+    for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads)
+      delta_particle<VAR, true, COLLIDE>(P, flags, g, Q, QX, lam, keys, cs, ce, Qo, QXo, DP, VS);
+    return;
+  }
+  tile_load<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, [&](uint32_t gi) { return lam[gi]; });
+  __syncthreads();
+  for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
+    const uint32_t key = keys[g];
+    int h;
+    uint32_t slot;
+    tile_locate<BITS>(sh, key, g, ix0, iy0, iz0, h, slot);
+    const uint32_t n = use_list ? (uint32_t)L.cnt[g] : (uint32_t)kNlInvalid;
+    if (n == kNlInvalid && tile_count<BITS>(sh, h) > P.cap) {
+      delta_particle<VAR, true, COLLIDE>(P, flags, g, Q, QX, lam, keys, cs, ce, Qo, QXo, DP, VS);
+      continue;
+    }
+    const float4 qb = s_q[slot];
+    DeltaAcc a;
+    const auto pair = [&](uint32_t j) {
+      const float4 qj = s_q[j];
+      delta_pair<VAR, true, COLLIDE, NS>(P, qb.w, qj.w, qb.x - qj.x, qb.y - qj.y, qb.z - qj.z, true, a);
+    };
+    if (n != kNlInvalid) nl_visit(L, g, n, pair);
+    else tile_stencil<BITS>(sh, h, slot, pair);
+    const float cx = cell_corner(P.origin[0], P.cell, compact10(key >> 2));
+    const float cy = cell_corner(P.origin[1], P.cell, compact10(key >> 1));
+    const float cz = cell_corner(P.origin[2], P.cell, compact10(key));
+    delta_finish<VAR, true>(P, flags, g, Q[g], QX, a, cx, cy, cz, Qo, QXo, DP, VS);
+  }
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileList T, NeighbourList L,
+                                                             const float4* __restrict__ Q,
+                                                             const float4* __restrict__ VS,
+                                                             const uint32_t* __restrict__ keys,
+                                                             const uint32_t* __restrict__ cs,
+                                                             const uint32_t* __restrict__ ce, float4* __restrict__ OM) {
+  constexpr int VAR = PBD_KERNEL_TEXTBOOK;
+  constexpr int CAP = TileCap<BITS>::value;
+  __shared__ TileShared<BITS> sh;
+  __shared__ float4 s_q[tile_lds<BITS>()];
+  __shared__ float4 s_v[CAP + 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_LIST_BUILDS], 1ull);
+  if (blockIdx.x >= *T.count) return;
+  int ix0, iy0, iz0;
+  const uint32_t total = tile_setup<BITS>(P, T.list[blockIdx.x], cs, ce, sh, ix0, iy0, iz0);
+  const uint32_t g0 = sh.g0, g1 = sh.g1;
+  if (total > (uint32_t)CAP) {
+    for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
+      omega_particle<VAR>(P, g, Q, VS, keys, cs, ce, OM);
+      L.cnt[g] = kNlInvalid;
+    }
+    if (threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_BLOCK], (unsigned long long)(g1 - g0));
+    return;
+  }
+  tile_load<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, [](uint32_t) { return 0.0f; });
+  for (uint32_t i = threadIdx.x; i < total; i += kTileThreads) {
+    const int c = tile_cell_of<BITS>(sh, i);
+    s_v[i] = VS[sh.gstart[c] + (i - sh.lstart[c])];
+  }
+  if (threadIdx.x == 0) s_v[CAP] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  __syncthreads();
+  for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
+    int h;
+    uint32_t slot;
+    tile_locate<BITS>(sh, keys[g], g, ix0, iy0, iz0, h, slot);
+    if (tile_count<BITS>(sh, h) > P.cap) {
+      omega_particle<VAR>(P, g, Q, VS, keys, cs, ce, OM);
+      L.cnt[g] = kNlInvalid;
+      atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_CAP], 1ull);
+      continue;
+    }
+    const float4 qi = s_q[slot], vi = s_v[slot];
+    OmegaAcc a;
+    const auto pair = [&](uint32_t j) {
+      const float4 qj = s_q[j], vj = s_v[j];
+      omega_pair<VAR>(P, qi.x - qj.x, qi.y - qj.y, qi.z - qj.z, vj.x - vi.x, vj.y - vi.y, vj.z - vi.z, a);
+    };
+    const uint32_t n = nl_build<BITS, CAP>(P, L, g, sh, s_q, h, slot, qi);
+    L.cnt[g] = (uint16_t)n;
+    if (n != kNlInvalid) {
+      nl_visit(L, g, n, pair);
+    } else {
+      tile_stencil<BITS>(sh, h, slot, pair);
+      atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_LONG], 1ull);
+    }
+    OM[g] = omega_finish<VAR>(P, a);
+  }
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kTileThreads) void k_tile_final(
+    DevParams P, TileList T, NeighbourList L, int use_list, uint32_t flags, const float4* __restrict__ Q,
+    const float4* __restrict__ QX, const float4* __restrict__ VS, const float4* __restrict__ OM,
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+    const float4* __restrict__ CS, const uint32_t* __restrict__ ncnt, float4* __restrict__ Xo,
+    float4* __restrict__ Po, float4* __restrict__ Vo, float4* __restrict__ Co, float4* __restrict__ Fo) {
+  constexpr int VAR = PBD_KERNEL_TEXTBOOK;
+  constexpr int CAP = TileCap<BITS>::value;
+  __shared__ TileShared<BITS> sh;
+  __shared__ float4 s_q[tile_lds<BITS>()];
+  __shared__ float4 s_v[CAP + 1];
+  if (blockIdx.x >= *T.count) return;
+  int ix0, iy0, iz0;
+  const uint32_t total = tile_setup<BITS>(P, T.list[blockIdx.x], cs, ce, sh, ix0, iy0, iz0);
+  const uint32_t g0 = sh.g0, g1 = sh.g1;
+  const bool vort = (flags & FF_VORT) != 0, xsph = (flags & FF_XSPH) != 0;
+  if (total > (uint32_t)CAP) {
+    for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads)
+      final_particle<VAR>(P, flags, g, Q, QX, VS, OM, keys, cs, ce, CS, ncnt, Xo, Po, Vo, Co, Fo);
+    return;
+  }
+  tile_load<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, [&](uint32_t gi) { return vort ? OM[gi].w : 0.0f; });
+  for (uint32_t i = threadIdx.x; i < total; i += kTileThreads) {
+    const int c = tile_cell_of<BITS>(sh, i);
+    s_v[i] = VS[sh.gstart[c] + (i - sh.lstart[c])];
+  }
+  if (threadIdx.x == 0) s_v[CAP] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  __syncthreads();
+  for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
+    const uint32_t key = keys[g];
+    int h;
+    uint32_t slot;
+    tile_locate<BITS>(sh, key, g, ix0, iy0, iz0, h, slot);
+    const uint32_t n = use_list ? (uint32_t)L.cnt[g] : (uint32_t)kNlInvalid;
+    if (n == kNlInvalid && tile_count<BITS>(sh, h) > P.cap) {
+      final_particle<VAR>(P, flags, g, Q, QX, VS, OM, keys, cs, ce, CS, ncnt, Xo, Po, Vo, Co, Fo);
+      continue;
+    }
+    const float4 qb = s_q[slot], vi = s_v[slot];
+    const float4 oi = vort ? OM[g] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    FinalAcc a;
+    if (vort || xsph) {
+      const auto pair = [&](uint32_t j) {
+        const float4 qj = s_q[j], vj = s_v[j];
+        final_pair_textbook<VAR>(P, vort, xsph, qb.x - qj.x, qb.y - qj.y, qb.z - qj.z, vi, vj, oi.w, qj.w, a);
+      };
+      if (n != kNlInvalid) nl_visit(L, g, n, pair);
+      else tile_stencil<BITS>(sh, h, slot, pair);
+    }
+    const float cx = cell_corner(P.origin[0], P.cell, compact10(key >> 2));
+    const float cy = cell_corner(P.origin[1], P.cell, compact10(key >> 1));
+    const float cz = cell_corner(P.origin[2], P.cell, compact10(key));
+    final_finish<VAR>(P, flags, g, Q[g], vi, oi, a, cx, cy, cz, QX, CS, ncnt, Xo, Po, Vo, Co, Fo);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+static NeighbourList nlist(const StepBuffers& B) { return NeighbourList{B.NL, B.NLC, B.nl_cap, B.STATS}; }
+
+hipError_t launch_tile_lambda(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
+                              const StepBuffers& B, const float4* Q, const uint32_t* keys, bool first) {
+  dim3 g(grid), b(kTileThreads);
+  const NeighbourList L = nlist(B);
+  if (bits == 5) {
+    if (first)
+      hipLaunchKernelGGL((k_tile_lambda<5, true>), g, b, 0, st, P, T, L, Q, keys, B.cstart, B.cend, B.LAM, B.DENS,
+                         B.NCNT, B.CCNT);
+    else
+      hipLaunchKernelGGL((k_tile_lambda<5, false>), g, b, 0, st, P, T, L, Q, keys, B.cstart, B.cend, B.LAM, B.DENS,
+                         B.NCNT, B.CCNT);
+  } else {
+    if (first)
+      hipLaunchKernelGGL((k_tile_lambda<6, true>), g, b, 0, st, P, T, L, Q, keys, B.cstart, B.cend, B.LAM, B.DENS,
+                         B.NCNT, B.CCNT);
+    else
+      hipLaunchKernelGGL((k_tile_lambda<6, false>), g, b, 0, st, P, T, L, Q, keys, B.cstart, B.cend, B.LAM, B.DENS,
+                         B.NCNT, B.CCNT);
+  }
+  return hipGetLastError();
+}
+
+template <int BITS, bool COLLIDE, int NS>
+static void launch_delta_t(hipStream_t st, uint32_t grid, const DevParams& P, const TileList& T,
+                           const StepBuffers& B, int use_list, uint32_t flags, const float4* Q, const float4* QX,
+                           float4* Qo, float4* QXo, const uint32_t* keys) {
+  hipLaunchKernelGGL((k_tile_delta<BITS, COLLIDE, NS>), dim3(grid), dim3(kTileThreads), 0, st, P, T, nlist(B),
+                     use_list, flags, Q, QX, B.LAM, keys, B.cstart, B.cend, Qo, QXo, B.DP, B.VS);
+}
+
+template <int BITS>
+static void launch_delta_b(hipStream_t st, uint32_t grid, const DevParams& P, const TileList& T,
+                           const StepBuffers& B, int use_list, uint32_t flags, bool collide, const float4* Q,
+                           const float4* QX, float4* Qo, float4* QXo, const uint32_t* keys) {
+  const bool n4 = P.n_scorr == 4;  // the s_corr exponent of Macklin & Mueller 2013
+  if (collide) {
+    if (n4) launch_delta_t<BITS, true, 4>(st, grid, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
+    else launch_delta_t<BITS, true, 0>(st, grid, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
+  } else {
+    if (n4) launch_delta_t<BITS, false, 4>(st, grid, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
+    else launch_delta_t<BITS, false, 0>(st, grid, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
+  }
+}
+
+hipError_t launch_tile_delta(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
+                             const StepBuffers& B, uint32_t flags, bool collide, bool use_list, const float4* Q,
+                             const float4* QX, float4* Qo, float4* QXo, const uint32_t* keys) {
+  const int ul = use_list ? 1 : 0;
+  if (bits == 5) launch_delta_b<5>(st, grid, P, T, B, ul, flags, collide, Q, QX, Qo, QXo, keys);
+  else launch_delta_b<6>(st, grid, P, T, B, ul, flags, collide, Q, QX, Qo, QXo, keys);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_omega(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
+                             const StepBuffers& B, const float4* Q, const uint32_t* keys) {
+  dim3 g(grid), b(kTileThreads);
+  const NeighbourList L = nlist(B);
+  if (bits == 5)
+    hipLaunchKernelGGL((k_tile_omega<5>), g, b, 0, st, P, T, L, Q, B.VS, keys, B.cstart, B.cend, B.OM);
+  else
+    hipLaunchKernelGGL((k_tile_omega<6>), g, b, 0, st, P, T, L, Q, B.VS, keys, B.cstart, B.cend, B.OM);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_final(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
+                             const StepBuffers& B, uint32_t flags, bool use_list, const float4* Q, const float4* QX,
+                             const uint32_t* keys) {
+  dim3 g(grid), b(kTileThreads);
+  const NeighbourList L = nlist(B);
+  const int ul = use_list ? 1 : 0;
+  if (bits == 5)
+    hipLaunchKernelGGL((k_tile_final<5>), g, b, 0, st, P, T, L, ul, flags, Q, QX, B.VS, B.OM, keys, B.cstart,
+                       B.cend, B.CS, B.NCNT, B.X, B.P, B.V, B.C, B.F);
+  else
+    hipLaunchKernelGGL((k_tile_final<6>), g, b, 0, st, P, T, L, ul, flags, Q, QX, B.VS, B.OM, keys, B.cstart,
+                       B.cend, B.CS, B.NCNT, B.X, B.P, B.V, B.C, B.F);
+  return hipGetLastError();
+}
+
+}  // namespace pbd

@@ -108,6 +108,7 @@ class PbdParams(C.Structure):
         ("grid_cell_size", C.c_float),
         ("grid_origin", _f3),
         ("grid_dims", _u3),
+        ("tile_bits", C.c_uint32),
     ]
 
     def copy(self) -> "PbdParams":
@@ -166,6 +167,8 @@ _SIGS = {
     "pbd_get_timings": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
                                   C.POINTER(C.c_uint64), C.c_uint32, C.POINTER(C.c_uint32)]),
     "pbd_reset_timings": (C.c_int, [C.c_void_p]),
+    "pbd_get_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "pbd_reset_counters": (C.c_int, [C.c_void_p]),
     "pbd_last_error": (C.c_char_p, [C.c_void_p]),
     "pbd_destroy": (None, [C.c_void_p]),
     "pbd_version": (C.c_int, []),
@@ -377,3 +380,15 @@ class Context:
         count = C.c_uint32(0)
         self._check(lib().pbd_get_timings(self._ptr, names, ms, launches, cap, C.byref(count)), "pbd_get_timings")
         return {names[i].decode(): (ms[i], launches[i]) for i in range(count.value)}
+
+    COUNTERS = ("list_builds", "unlisted_long", "unlisted_cap", "unlisted_block")
+
+    def counters(self) -> dict:
+        """Work counters of the neighbour passes: list builds and particle-builds left without a list, by cause."""
+        vals = (C.c_uint64 * 8)()
+        count = C.c_uint32(0)
+        self._check(lib().pbd_get_counters(self._ptr, vals, 8, C.byref(count)), "pbd_get_counters")
+        return {self.COUNTERS[i]: int(vals[i]) for i in range(min(count.value, len(self.COUNTERS)))}
+
+    def reset_counters(self) -> None:
+        self._check(lib().pbd_reset_counters(self._ptr), "pbd_reset_counters")
