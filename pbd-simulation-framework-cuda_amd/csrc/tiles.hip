@@ -228,6 +228,24 @@ __device__ __forceinline__ void tile_load(const DevParams& P, const TileShared<B
   if (threadIdx.x < 4) s_q[CAP + threadIdx.x] = make_float4(kFar, kFar, kFar, 0.0f);
 }
 
+// tile_load plus the global index of every slot in s_gi (the sentinel slot CAP
+// maps to `any`, a valid particle whose pair terms the sentinel's position zeroes).
+template <int BITS, int CAP, class Payload>
+__device__ __forceinline__ void tile_load_gi(const DevParams& P, const TileShared<BITS>& sh, uint32_t total,
+                                             const float4* __restrict__ Q, int ix0, int iy0, int iz0, float4* s_q,
+                                             uint32_t* s_gi, uint32_t any, Payload payload) {
+  for (uint32_t i = threadIdx.x; i < total; i += kTileThreads) {
+    const int c = tile_cell_of<BITS>(sh, i);
+    const uint32_t gi = sh.gstart[c] + (i - sh.lstart[c]);
+    const float4 q = Q[gi];
+    const float3 o = tile_offset<BITS>(P, c, ix0, iy0, iz0);
+    s_q[i] = make_float4(q.x + o.x, q.y + o.y, q.z + o.z, payload(gi));
+    s_gi[i] = gi;
+  }
+  if (threadIdx.x < 4) s_q[CAP + threadIdx.x] = make_float4(kFar, kFar, kFar, 0.0f);
+  if (threadIdx.x == 0) s_gi[CAP] = any;
+}
+
 // Visit the global list of particle g (n = padded length) in order.
 template <class Body>
 __device__ __forceinline__ void nl_visit(const NeighbourList& L, uint32_t g, uint32_t n, Body body) {
@@ -378,7 +396,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_lambda(DevParams P, TileL
     }
     const float4 qi = s_q[slot];
     LambdaAcc a;
-    const auto pair = [&](uint32_t j) {
+    const auto pair = [&](uint32_t j) __attribute__((always_inline)) {
       const float4 qj = s_q[j];
       lambda_pair<VAR>(P, qi.x - qj.x, qi.y - qj.y, qi.z - qj.z, a);
     };
@@ -439,7 +457,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_delta(DevParams P, TileLi
     }
     const float4 qb = s_q[slot];
     DeltaAcc a;
-    const auto pair = [&](uint32_t j) {
+    const auto pair = [&](uint32_t j) __attribute__((always_inline)) {
       const float4 qj = s_q[j];
       delta_pair<VAR, true, COLLIDE, NS>(P, qb.w, qj.w, qb.x - qj.x, qb.y - qj.y, qb.z - qj.z, true, a);
     };
@@ -463,7 +481,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileLi
   constexpr int CAP = TileCap<BITS>::value;
   __shared__ TileShared<BITS> sh;
   __shared__ float4 s_q[tile_lds<BITS>()];
-  __shared__ float4 s_v[CAP + 1];
+  __shared__ uint32_t s_gi[CAP + 1];  // global index of each slot: velocities are gathered from L2
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_LIST_BUILDS], 1ull);
   if (blockIdx.x >= *T.count) return;
   int ix0, iy0, iz0;
@@ -477,12 +495,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileLi
     if (threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_BLOCK], (unsigned long long)(g1 - g0));
     return;
   }
-  tile_load<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, [](uint32_t) { return 0.0f; });
-  for (uint32_t i = threadIdx.x; i < total; i += kTileThreads) {
-    const int c = tile_cell_of<BITS>(sh, i);
-    s_v[i] = VS[sh.gstart[c] + (i - sh.lstart[c])];
-  }
-  if (threadIdx.x == 0) s_v[CAP] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  tile_load_gi<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, s_gi, g0, [](uint32_t) { return 0.0f; });
   __syncthreads();
   for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
     int h;
@@ -494,10 +507,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileLi
       atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_CAP], 1ull);
       continue;
     }
-    const float4 qi = s_q[slot], vi = s_v[slot];
+    const float4 qi = s_q[slot], vi = VS[g];
     OmegaAcc a;
-    const auto pair = [&](uint32_t j) {
-      const float4 qj = s_q[j], vj = s_v[j];
+    const auto pair = [&](uint32_t j) __attribute__((always_inline)) {
+      const float4 qj = s_q[j], vj = VS[s_gi[j]];
       omega_pair<VAR>(P, qi.x - qj.x, qi.y - qj.y, qi.z - qj.z, vj.x - vi.x, vj.y - vi.y, vj.z - vi.z, a);
     };
     const uint32_t n = nl_build<BITS, CAP>(P, L, g, sh, s_q, h, slot, qi);
@@ -512,7 +525,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileLi
   }
 }
 
-template <int BITS>
+template <int BITS, bool VORT, bool XSPH>
 __global__ __launch_bounds__(kTileThreads) void k_tile_final(
     DevParams P, TileList T, NeighbourList L, int use_list, uint32_t flags, const float4* __restrict__ Q,
     const float4* __restrict__ QX, const float4* __restrict__ VS, const float4* __restrict__ OM,
@@ -523,23 +536,19 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_final(
   constexpr int CAP = TileCap<BITS>::value;
   __shared__ TileShared<BITS> sh;
   __shared__ float4 s_q[tile_lds<BITS>()];
-  __shared__ float4 s_v[CAP + 1];
+  __shared__ uint32_t s_gi[CAP + 1];  // global index of each slot: velocities are gathered from L2
   if (blockIdx.x >= *T.count) return;
   int ix0, iy0, iz0;
   const uint32_t total = tile_setup<BITS>(P, T.list[blockIdx.x], cs, ce, sh, ix0, iy0, iz0);
   const uint32_t g0 = sh.g0, g1 = sh.g1;
-  const bool vort = (flags & FF_VORT) != 0, xsph = (flags & FF_XSPH) != 0;
+  constexpr bool vort = VORT, xsph = XSPH;  // flags & FF_VORT, flags & FF_XSPH
   if (total > (uint32_t)CAP) {
     for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads)
       final_particle<VAR>(P, flags, g, Q, QX, VS, OM, keys, cs, ce, CS, ncnt, Xo, Po, Vo, Co, Fo);
     return;
   }
-  tile_load<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, [&](uint32_t gi) { return vort ? OM[gi].w : 0.0f; });
-  for (uint32_t i = threadIdx.x; i < total; i += kTileThreads) {
-    const int c = tile_cell_of<BITS>(sh, i);
-    s_v[i] = VS[sh.gstart[c] + (i - sh.lstart[c])];
-  }
-  if (threadIdx.x == 0) s_v[CAP] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  tile_load_gi<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, s_gi, g0,
+                          [&](uint32_t gi) { return vort ? OM[gi].w : 0.0f; });
   __syncthreads();
   for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
     const uint32_t key = keys[g];
@@ -551,13 +560,19 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_final(
       final_particle<VAR>(P, flags, g, Q, QX, VS, OM, keys, cs, ce, CS, ncnt, Xo, Po, Vo, Co, Fo);
       continue;
     }
-    const float4 qb = s_q[slot], vi = s_v[slot];
+    const float4 qb = s_q[slot];
+    // velocity components as scalars: copies of the float4 aggregate inside the
+    // pair loop are not promoted to registers and end up in scratch
+    const float4 v4 = VS[g];
+    const float vx = v4.x, vy = v4.y, vz = v4.z, vw = v4.w;
     const float4 oi = vort ? OM[g] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     FinalAcc a;
     if (vort || xsph) {
-      const auto pair = [&](uint32_t j) {
-        const float4 qj = s_q[j], vj = s_v[j];
-        final_pair_textbook<VAR>(P, vort, xsph, qb.x - qj.x, qb.y - qj.y, qb.z - qj.z, vi, vj, oi.w, qj.w, a);
+      const auto pair = [&](uint32_t j) __attribute__((always_inline)) {
+        const float4 qj = s_q[j];
+        const float4 vj = XSPH ? VS[s_gi[j]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // unused without XSPH
+        final_pair_textbook<VAR>(P, vort, xsph, qb.x - qj.x, qb.y - qj.y, qb.z - qj.z, make_float4(vx, vy, vz, vw),
+                                 vj, oi.w, qj.w, a);
       };
       if (n != kNlInvalid) nl_visit(L, g, n, pair);
       else tile_stencil<BITS>(sh, h, slot, pair);
@@ -565,7 +580,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_final(
     const float cx = cell_corner(P.origin[0], P.cell, compact10(key >> 2));
     const float cy = cell_corner(P.origin[1], P.cell, compact10(key >> 1));
     const float cz = cell_corner(P.origin[2], P.cell, compact10(key));
-    final_finish<VAR>(P, flags, g, Q[g], vi, oi, a, cx, cy, cz, QX, CS, ncnt, Xo, Po, Vo, Co, Fo);
+    final_finish<VAR>(P, flags, g, Q[g], make_float4(vx, vy, vz, vw), oi, a, cx, cy, cz, QX, CS, ncnt, Xo, Po, Vo,
+                      Co, Fo);
   }
 }
 
@@ -643,12 +659,22 @@ hipError_t launch_tile_final(hipStream_t st, int bits, uint32_t grid, const DevP
   dim3 g(grid), b(kTileThreads);
   const NeighbourList L = nlist(B);
   const int ul = use_list ? 1 : 0;
-  if (bits == 5)
-    hipLaunchKernelGGL((k_tile_final<5>), g, b, 0, st, P, T, L, ul, flags, Q, QX, B.VS, B.OM, keys, B.cstart,
-                       B.cend, B.CS, B.NCNT, B.X, B.P, B.V, B.C, B.F);
-  else
-    hipLaunchKernelGGL((k_tile_final<6>), g, b, 0, st, P, T, L, ul, flags, Q, QX, B.VS, B.OM, keys, B.cstart,
-                       B.cend, B.CS, B.NCNT, B.X, B.P, B.V, B.C, B.F);
+  const bool vort = (flags & FF_VORT) != 0, xsph = (flags & FF_XSPH) != 0;
+#define PBD_FINAL(BITS_, V_, X_)                                                                                \
+  hipLaunchKernelGGL((k_tile_final<BITS_, V_, X_>), g, b, 0, st, P, T, L, ul, flags, Q, QX, B.VS, B.OM, keys,   \
+                     B.cstart, B.cend, B.CS, B.NCNT, B.X, B.P, B.V, B.C, B.F)
+  if (bits == 5) {
+    if (vort && xsph) PBD_FINAL(5, true, true);
+    else if (vort) PBD_FINAL(5, true, false);
+    else if (xsph) PBD_FINAL(5, false, true);
+    else PBD_FINAL(5, false, false);
+  } else {
+    if (vort && xsph) PBD_FINAL(6, true, true);
+    else if (vort) PBD_FINAL(6, true, false);
+    else if (xsph) PBD_FINAL(6, false, true);
+    else PBD_FINAL(6, false, false);
+  }
+#undef PBD_FINAL
   return hipGetLastError();
 }
 
