@@ -48,6 +48,35 @@ def stage_bytes(stage, n, p, table_size, passes):
     return per.get(stage, 0) * n
 
 
+def measured_traffic(kernel_stage):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_pbf_step_dam_break_1m.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 4 and w[0] == "stage" and w[1] == kernel_stage and w[2] == "hbm_bytes_per_launch":
+                    return float(w[3])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    """Three copies of 256 MiB between device buffers (torch's copy kernel)."""
+    import torch
+
+    nbytes = 256 << 20
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+
+
 def cpu_baseline(p, steps):
     """The CPU oracle (FP64 restatement of the same step, OpenMP) timed on this host."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -78,9 +107,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="dam_break_1m")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--profile-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device-to-device copy (the known-size kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters to HBM bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the step runs on a single "
               f"MI355X; the multi-GPU decomposition is a design note only)", file=sys.stderr)
@@ -88,45 +123,39 @@ def main():
 
     import pbdfluid as F
 
-    p = F.load_params(F.config_path(args.config))
-    p.box_spawn_in_the_beginning = 1
+    frame = F.load_params(F.config_path(args.config))
+    frame.box_spawn_in_the_beginning = 1
+    # one step = one substep (one Fluid::compute) of the frame's dt / substeps
+    p = frame.copy()
+    p.delta_t = frame.delta_t / frame.substeps
+    p.substeps = 1
     ctx = F.Context(p)
     n = ctx.n
     table = int(F.capi.lib().pbd_morton_encode(p.grid_dims[0] - 1, p.grid_dims[1] - 1, p.grid_dims[2] - 1)) + 1
     passes = max(1, (table - 1).bit_length() + 7 >> 3)
 
-    for _ in range(args.warmup):
-        ctx.compute()
+    # warm-up, then the timed region: K steps, each one launch of the captured
+    # step graph on the context's stream, bracketed by stream synchronisation
+    ctx.step(args.warmup, graph=True)
     ctx.sync()
-
-    # timed region: K substeps, per-kernel HIP events on the context's stream
-    ctx.set_timing(True)
-    ctx.reset_timings()
     ctx.reset_counters()
     ctx.sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.compute()
+    ctx.step(args.steps, graph=True)
     ctx.sync()
     elapsed = time.perf_counter() - t0
-    timings = ctx.timings()
     counters = ctx.counters()
-    ctx.set_timing(False)
     value = n * args.steps / elapsed
 
-    # the same K substeps replayed as a captured hipGraph (no events)
-    graph_value = None
-    if not args.no_graph:
-        q = p.copy()
-        q.delta_t = p.delta_t / p.substeps  # one graph replay = one substep of the same dt
-        q.substeps = 1
-        ctx.set_params(q)
-        ctx.step(2, graph=True)
-        ctx.sync()
-        t0 = time.perf_counter()
-        ctx.step(args.steps, graph=True)
-        ctx.sync()
-        graph_value = n * args.steps / (time.perf_counter() - t0)
+    # per-kernel durations: the same step launched kernel by kernel with a HIP
+    # event pair around every launch on the context's stream
+    ki = max(1, min(args.steps, args.profile_steps))
+    ctx.set_timing(True)
+    ctx.reset_timings()
+    ctx.step(ki)
+    ctx.sync()
+    timings = ctx.timings()
+    ctx.set_timing(False)
     bad = ctx.check_finite()
     ctx.close()
 
@@ -139,6 +168,7 @@ def main():
     for k, v in stages.items():
         b = stage_bytes(k, n, p, table, passes)
         v["algorithmic_GBps"] = (b / (v["us_per_launch"] * 1e-6) / 1e9) if v["us_per_launch"] else 0.0
+    traffic = measured_traffic(dom)
 
     line = {
         "metric": METRIC,
@@ -156,6 +186,7 @@ def main():
         "config": {
             "workload": f"{args.config}: {n}-particle PBF dam break (64x128x128 lattice, spacing 0.05, "
                         "h = cell = 0.1, 4 Jacobi iterations, 1 stabilization pass, dt = 1/120 per step)",
+            "step": "one substep (Fluid::compute), one hipGraph launch",
             "particles": n,
             "solver_iterations": p.solver_iterations,
             "kernel_variant": "textbook" if p.kernel_variant == 1 else "reference",
@@ -168,10 +199,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
         },
-        "graph_value": graph_value,
         "steps_per_s": args.steps / elapsed,
+        "profiled_steps": ki,
         "stages": stages,
         "nonfinite_particles": bad,
         "neighbour_lists": counters,

@@ -56,13 +56,18 @@ int Fluid::alloc_table() {
   const uint32_t ts = host_morton(params_.grid_dims[0] - 1u, params_.grid_dims[1] - 1u, params_.grid_dims[2] - 1u) + 1u;
   if (B_.cstart && ts == B_.table_size) return PBD_OK;
   if (B_.cstart) {
-    for (void* q : {(void*)B_.cstart, (void*)B_.cend}) {
+    for (void* q : {(void*)B_.cstart, (void*)B_.cend, (void*)tile_flag_, (void*)tile_bcount_}) {
       (void)hipFree(q);
       allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), q), allocations_.end());
     }
     B_.cstart = B_.cend = nullptr;
+    tile_flag_ = tile_bcount_ = nullptr;
   }
-  if (dev_alloc(allocations_, &B_.cstart, ts) || dev_alloc(allocations_, &B_.cend, ts))
+  // tile flags for the smallest tile (2^5 cells) and the compaction's block counts
+  const uint32_t nt = ((ts - 1u) >> 5) + 1u;
+  if (dev_alloc(allocations_, &B_.cstart, ts) || dev_alloc(allocations_, &B_.cend, ts) ||
+      dev_alloc(allocations_, &tile_flag_, nt) ||
+      dev_alloc(allocations_, &tile_bcount_, (nt + kTileCompactChunk - 1u) / kTileCompactChunk))
     return fail(PBD_ERR_NOMEM, "cell table allocation failed");
   B_.table_size = ts;
   sort_.passes = sort_passes_for(ts);
@@ -287,10 +292,11 @@ int Fluid::enqueue_substep() {
   // LDS tiles for the 27-cell stencil of the textbook kernels (tiles.hip)
   const int bits = (int)params_.tile_bits;
   const bool tiles = bits != 0 && P.variant == PBD_KERNEL_TEXTBOOK && P.kw == 1 && (g & G_FIND_CONTACTS);
-  uint32_t tgrid = 0;
+  uint32_t tgrid = 0, ntiles = 0;
   if (tiles) {
-    const uint32_t blocks = ((B_.table_size - 1u) >> bits) + 1u;
-    tgrid = std::min(blocks, P.n);
+    ntiles = ((B_.table_size - 1u) >> bits) + 1u;
+    // a multiple of XCDs x tile chunk, so that tile_rank maps the grid onto [0, tgrid)
+    tgrid = (std::min(ntiles, P.n) + kTileGridMultiple - 1u) / kTileGridMultiple * kTileGridMultiple;
   }
   const TileList TL{tile_list_, tile_count_};
   // 1-2. applyForces + initializeCellIds (Fluid.cu:21-25)
@@ -298,12 +304,13 @@ int Fluid::enqueue_substep() {
   // resetCellInfo (Fluid.cu:31): empty = {UINT_MAX, N}
   PBD_LAUNCH(ST_CELL_RESET, launch_fill_u32(st, B_.cstart, 0xFFFFFFFFu, B_.table_size));
   PBD_LAUNCH(ST_CELL_RESET, launch_fill_u32(st, B_.cend, P.n, B_.table_size));
-  if (tiles) PBD_LAUNCH(ST_CELL_RESET, launch_fill_u32(st, tile_count_, 0u, 1u));
+  if (tiles) PBD_LAUNCH(ST_CELL_RESET, launch_fill_u32(st, tile_flag_, 0u, ntiles));
   // 3. sortIds (Fluid.cu:27)
   PBD_LAUNCH(ST_SORT, launch_sort(st, sort_, P.n, B_.keys_in, &keys_sorted_, &ids_sorted_));
   // 4-5. reorderStorage + computeCellInfo (Fluid.cu:29,33)
   PBD_LAUNCH(ST_REORDER, launch_reorder(st, tpb, P, B_, keys_sorted_, ids_sorted_, tiles ? (uint32_t)bits : 0u,
-                                        tile_list_, tile_count_));
+                                        tile_flag_));
+  if (tiles) PBD_LAUNCH(ST_REORDER, launch_tile_compact(st, ntiles, tile_flag_, tile_bcount_, tile_list_, tile_count_));
   const uint32_t* keys = keys_sorted_;
 
   // 6-7. solver loop (Fluid.cu:49-80); contacts/neighbours are walked inside the passes

@@ -90,8 +90,10 @@ class Fluid : public Cuda_Computable {
   const uint32_t* keys_sorted_ = nullptr;
   const uint32_t* ids_sorted_ = nullptr;
   uint32_t* d_scratch_ = nullptr;  // small device scratch: [0] finite count, [1] unused
-  uint32_t* tile_list_ = nullptr;  // occupied Morton blocks of the step
+  uint32_t* tile_list_ = nullptr;  // occupied Morton blocks of the step, ascending
   uint32_t* tile_count_ = nullptr;
+  uint32_t* tile_flag_ = nullptr;    // per Morton block: occupied this step
+  uint32_t* tile_bcount_ = nullptr;  // compaction block counts
 
   // graph of one substep
   hipGraph_t graph_ = nullptr;

@@ -44,17 +44,29 @@ struct StepBuffers {
   unsigned long long* STATS;  // work counters (PBD_COUNTER_*)
 };
 
-// Occupied Morton blocks of the current step (built by the reorder kernel).
+// Occupied Morton blocks of the current step in Morton order (the reorder
+// kernel flags them, launch_tile_compact lists them).
 struct TileList {
-  const uint32_t* list;   // block ids (cell id >> bits), unordered
+  const uint32_t* list;   // block ids (cell id >> bits), ascending
   const uint32_t* count;  // number of entries
 };
 
+constexpr uint32_t kTileCompactChunk = 4096;  // flags per block of the compaction
+#ifndef PBD_TILE_CHUNK
+#define PBD_TILE_CHUNK 4
+#endif
+constexpr uint32_t kTileChunk = PBD_TILE_CHUNK;        // consecutive tiles per XCD turn (tiles.hip tile_rank)
+constexpr uint32_t kTileGridMultiple = 8 * kTileChunk;  // tile grids are a multiple of (XCDs x tile chunk)
+// list / *count <- the indices of the set flags among flags[0, nflags), ascending;
+// bcount holds ceil(nflags / kTileCompactChunk) block counts.
+hipError_t launch_tile_compact(hipStream_t st, uint32_t nflags, const uint32_t* flags, uint32_t* bcount,
+                               uint32_t* list, uint32_t* count);
+
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n);
 hipError_t launch_predict_keys(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B);
+// tile_flag[b] = 1 for every occupied Morton block b of 2^tile_bits cells (tile_bits != 0)
 hipError_t launch_reorder(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,
-                          const uint32_t* keys_s, const uint32_t* ids_s, uint32_t tile_bits, uint32_t* tile_list,
-                          uint32_t* tile_count);
+                          const uint32_t* keys_s, const uint32_t* ids_s, uint32_t tile_bits, uint32_t* tile_flag);
 hipError_t launch_lambda(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, const float4* Q,
                          const uint32_t* keys_s, bool first);
 hipError_t launch_count(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,

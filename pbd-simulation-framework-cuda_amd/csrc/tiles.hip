@@ -166,6 +166,97 @@ __device__ __forceinline__ uint32_t tile_count(const TileShared<BITS>& sh, int h
 }
 
 // ---------------------------------------------------------------------------
+// Block -> tile.  Workgroups are dispatched to the 8 XCDs of the MI355X round
+// robin, and each XCD has its own L2.  Tile kernels deal the Morton-ordered tile
+// list to the XCDs in chunks of kTileChunk consecutive tiles (chunk c to XCD
+// c mod 8), so that the halos of neighbouring tiles (each particle is in ~4.5
+// halos) are mostly fetched once per L2 while dense and sparse regions still
+// spread evenly over the XCDs.  The grid is a multiple of kXcds * kTileChunk;
+// the map from block to rank is a bijection on [0, grid).
+constexpr uint32_t kXcds = 8;
+static_assert(kTileGridMultiple % (kXcds * kTileChunk) == 0, "tile grid multiple");
+
+// Rank in the tile list of this block's tile, or UINT32_MAX if the block has none.
+__device__ __forceinline__ uint32_t tile_rank(const TileList& T) {
+  const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
+  const uint32_t r = ((k / kTileChunk) * kXcds + x) * kTileChunk + k % kTileChunk;
+  return r < *T.count ? r : 0xFFFFFFFFu;
+}
+
+// ---------------------------------------------------------------------------
+// Tile-list compaction: a two-kernel stream compaction of the tile flags.
+__global__ __launch_bounds__(1024) void k_tile_count(uint32_t nflags, const uint32_t* __restrict__ flags,
+                                                     uint32_t* __restrict__ bcount) {
+  __shared__ uint32_t s_w[16];
+  const uint32_t base = blockIdx.x * kTileCompactChunk + threadIdx.x * 4u;
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) c += (base + k < nflags && flags[base + k]) ? 1u : 0u;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63u) == 0) s_w[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < 16; ++w) t += s_w[w];
+    bcount[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_tile_emit(uint32_t nflags, const uint32_t* __restrict__ flags,
+                                                    const uint32_t* __restrict__ bcount, uint32_t* __restrict__ list,
+                                                    uint32_t* __restrict__ count) {
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_off;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  // offset of this block: the counts of the blocks before it
+  uint32_t o = 0;
+  for (uint32_t i = tid; i < blockIdx.x; i += 1024u) o += bcount[i];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) o += __shfl_xor(o, d);
+  if (lane == 0) s_w[wave] = o;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < 16; ++w) t += s_w[w];
+    s_off = t;
+  }
+  __syncthreads();
+  // this thread's flags and their exclusive scan over the block
+  const uint32_t base = blockIdx.x * kTileCompactChunk + tid * 4u;
+  uint32_t f[4], c = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    f[k] = (base + k < nflags && flags[base + k]) ? 1u : 0u;
+    c += f[k];
+  }
+  uint32_t x = c;
+#pragma unroll
+  for (uint32_t d = 1; d < 64u; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  __syncthreads();
+  if (lane == 63u) s_w[wave] = x;
+  __syncthreads();
+  uint32_t pre = s_off;
+  for (uint32_t w = 0; w < wave; ++w) pre += s_w[w];
+  uint32_t pos = pre + x - c;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k)
+    if (f[k]) list[pos++] = base + k;
+  if (blockIdx.x == gridDim.x - 1 && tid == 1023u) *count = pre + x;
+}
+
+hipError_t launch_tile_compact(hipStream_t st, uint32_t nflags, const uint32_t* flags, uint32_t* bcount,
+                               uint32_t* list, uint32_t* count) {
+  const uint32_t blocks = (nflags + kTileCompactChunk - 1u) / kTileCompactChunk;
+  hipLaunchKernelGGL(k_tile_count, dim3(blocks), dim3(1024), 0, st, nflags, flags, bcount);
+  hipLaunchKernelGGL(k_tile_emit, dim3(blocks), dim3(1024), 0, st, nflags, flags, bcount, list, count);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Neighbour lists.
 //
 // The lambda pass scans the 27-cell stencil of each particle with a bare
@@ -369,9 +460,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_lambda(DevParams P, TileL
   __shared__ TileShared<BITS> sh;
   __shared__ float4 s_q[tile_lds<BITS>()];
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_LIST_BUILDS], 1ull);
-  if (blockIdx.x >= *T.count) return;
+  const uint32_t rank = tile_rank(T);
+  if (rank == 0xFFFFFFFFu) return;
   int ix0, iy0, iz0;
-  const uint32_t total = tile_setup<BITS>(P, T.list[blockIdx.x], cs, ce, sh, ix0, iy0, iz0);
+  const uint32_t total = tile_setup<BITS>(P, T.list[rank], cs, ce, sh, ix0, iy0, iz0);
   const uint32_t g0 = sh.g0, g1 = sh.g1;
   if (total > (uint32_t)CAP) {
     for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
@@ -434,9 +526,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_delta(DevParams P, TileLi
   constexpr int CAP = TileCap<BITS>::value;
   __shared__ TileShared<BITS> sh;
   __shared__ float4 s_q[tile_lds<BITS>()];
-  if (blockIdx.x >= *T.count) return;
+  const uint32_t rank = tile_rank(T);
+  if (rank == 0xFFFFFFFFu) return;
   int ix0, iy0, iz0;
-  const uint32_t total = tile_setup<BITS>(P, T.list[blockIdx.x], cs, ce, sh, ix0, iy0, iz0);
+  const uint32_t total = tile_setup<BITS>(P, T.list[rank], cs, ce, sh, ix0, iy0, iz0);
   const uint32_t g0 = sh.g0, g1 = sh.g1;
   if (total > (uint32_t)CAP) {
     for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads)
@@ -483,9 +576,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileLi
   __shared__ float4 s_q[tile_lds<BITS>()];
   __shared__ uint32_t s_gi[CAP + 1];  // global index of each slot: velocities are gathered from L2
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_LIST_BUILDS], 1ull);
-  if (blockIdx.x >= *T.count) return;
+  const uint32_t rank = tile_rank(T);
+  if (rank == 0xFFFFFFFFu) return;
   int ix0, iy0, iz0;
-  const uint32_t total = tile_setup<BITS>(P, T.list[blockIdx.x], cs, ce, sh, ix0, iy0, iz0);
+  const uint32_t total = tile_setup<BITS>(P, T.list[rank], cs, ce, sh, ix0, iy0, iz0);
   const uint32_t g0 = sh.g0, g1 = sh.g1;
   if (total > (uint32_t)CAP) {
     for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
@@ -537,9 +631,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_final(
   __shared__ TileShared<BITS> sh;
   __shared__ float4 s_q[tile_lds<BITS>()];
   __shared__ uint32_t s_gi[CAP + 1];  // global index of each slot: velocities are gathered from L2
-  if (blockIdx.x >= *T.count) return;
+  const uint32_t rank = tile_rank(T);
+  if (rank == 0xFFFFFFFFu) return;
   int ix0, iy0, iz0;
-  const uint32_t total = tile_setup<BITS>(P, T.list[blockIdx.x], cs, ce, sh, ix0, iy0, iz0);
+  const uint32_t total = tile_setup<BITS>(P, T.list[rank], cs, ce, sh, ix0, iy0, iz0);
   const uint32_t g0 = sh.g0, g1 = sh.g1;
   constexpr bool vort = VORT, xsph = XSPH;  // flags & FF_VORT, flags & FF_XSPH
   if (total > (uint32_t)CAP) {
