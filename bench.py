@@ -30,6 +30,8 @@ def stage_bytes(stage, n, p, table_size, passes):
         "sort": 4 + 12 + (passes - 1) * 16,
         # R sorted key + id (8), gather x, v, f_ext (48), W q, qx, v (48), colours when not recomputed (32)
         "reorder": 104 + (0 if gates_fn else 32),
+        # R the occupied-block flags twice (count, emit), W the block list: per flag, not per particle
+        "tile_list": 0,
         # R q (16), cell id (4); W lambda (4) [+ density, counts on the first pass]
         "lambda": 24,
         # R q, qx (32), lambda (4), cell id (4); W q', qx' (32)
@@ -106,7 +108,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="dam_break_1m")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--profile-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -7,8 +7,8 @@
 
 namespace pbd {
 
-const char* const kStageNames[ST_NUM] = {"predict_keys", "cell_reset", "sort",  "reorder", "lambda", "delta",
-                                         "collide",      "update",     "count", "omega",   "final"};
+const char* const kStageNames[ST_NUM] = {"predict_keys", "cell_reset", "sort",  "reorder", "tile_list", "lambda",
+                                         "delta",        "collide",    "update", "count",  "omega",     "final"};
 
 Fluid::~Fluid() {
   if (stream_) (void)hipStreamSynchronize(stream_);
@@ -310,7 +310,7 @@ int Fluid::enqueue_substep() {
   // 4-5. reorderStorage + computeCellInfo (Fluid.cu:29,33)
   PBD_LAUNCH(ST_REORDER, launch_reorder(st, tpb, P, B_, keys_sorted_, ids_sorted_, tiles ? (uint32_t)bits : 0u,
                                         tile_flag_));
-  if (tiles) PBD_LAUNCH(ST_REORDER, launch_tile_compact(st, ntiles, tile_flag_, tile_bcount_, tile_list_, tile_count_));
+  if (tiles) PBD_LAUNCH(ST_TILES, launch_tile_compact(st, ntiles, tile_flag_, tile_bcount_, tile_list_, tile_count_));
   const uint32_t* keys = keys_sorted_;
 
   // 6-7. solver loop (Fluid.cu:49-80); contacts/neighbours are walked inside the passes

@@ -123,6 +123,7 @@ enum Stage {
   ST_CELL_RESET,
   ST_SORT,
   ST_REORDER,
+  ST_TILES,
   ST_LAMBDA,
   ST_DELTA,
   ST_COLLIDE,
