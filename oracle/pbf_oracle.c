@@ -527,6 +527,65 @@ static double pass_collide(const orc_work* w, double* P, double* X, double* tmp)
   return minm;
 }
 
+/* collisionType 1: contact batches (Collision.h:291-464, Fluid.cu:62-71).
+ *
+ * The contacts are the rows of the neighbour table: the list orc_neighbours
+ * returns for each particle (27-cell contacts, then shells, capped), contact
+ * c = i * cap + k joining i and its k-th entry.  One stabilization iteration
+ * clears the solved flags (resetContactConstraintSuccess) and runs
+ * maxCollisionBatches batches.  In a batch every unsolved contact claims its two
+ * particles (setupCollisionConstraintBatches); a contact holding both is marked
+ * solved and, if the particles overlap (len < diameter, len > 1e-5), pushes them
+ * apart by half the overlap times (1 - stiffness) each, in p* and in x
+ * (setupCollisionConstraintBatchesCheck).
+ *
+ * The reference serialises the claims in index order within a thread block and
+ * lets blocks race for a particle.  Deterministic definition used here: a
+ * contact wins a particle when its index is the smallest among the unsolved
+ * contacts at that particle (the in-block order applied across blocks).  The
+ * contacts winning both particles form a matching, so their pushes are
+ * independent and applied in place. */
+static void pass_collide_batches(const orc_work* w, double* P, double* X, const uint32_t* nbl,
+                                 const uint32_t* nbc, uint8_t* solved, uint64_t* minc) {
+  const orc_derived* d = w->d;
+  const uint64_t cap = d->cap;
+  const uint32_t n = w->n;
+  memset(solved, 0, (size_t)n * cap);
+  for (uint32_t b = 0; b < d->max_batches; ++b) {
+    for (uint32_t i = 0; i < n; ++i) minc[i] = UINT64_MAX;
+    for (uint32_t i = 0; i < n; ++i)
+      for (uint32_t k = 0; k < nbc[i]; ++k) {
+        const uint64_t c = (uint64_t)i * cap + k;
+        if (solved[c]) continue;
+        const uint32_t j = nbl[c];
+        if (c < minc[i]) minc[i] = c;
+        if (c < minc[j]) minc[j] = c;
+      }
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint64_t c = minc[i];
+      if (c == UINT64_MAX || c / cap != i) continue;
+      const uint32_t j = nbl[c];
+      if (minc[j] != c) continue;
+      solved[c] = 1;
+      double dir[3];
+      sub3(&P[4 * j], &P[4 * i], dir);
+      const double len = len3(dir);
+      const double half = (d->diameter - len) / 2.0;
+      if (half > 0.0 && len > 0.00001) {
+        const double s = half * (1.0 - d->stiffness) / len;
+        for (int a = 0; a < 3; ++a) {
+          P[4 * i + a] -= s * dir[a];
+          X[4 * i + a] -= s * dir[a];
+          P[4 * j + a] += s * dir[a];
+          X[4 * j + a] += s * dir[a];
+        }
+        P[4 * i + 3] = 0.0; /* the pushed p* is written back with w = 0 (Collision.h:403,408,428-429) */
+        P[4 * j + 3] = 0.0;
+      }
+    }
+  }
+}
+
 /* computeOmega (Density.h:232-269). Textbook: omega_i = sum v_ij x grad_j W_ij. */
 static void pass_omega(const orc_work* w, const double* P, const double* V, double* Om) {
   const orc_derived* d = w->d;
@@ -698,6 +757,9 @@ int orc_step(const pbd_params* prm, orc_io* io) {
   uint32_t* cnt = (uint32_t*)malloc((size_t)n * 4);
   uint32_t* ccnt = (uint32_t*)malloc((size_t)n * 4);
   double* dens = (double*)calloc(n, sizeof(double));
+  uint32_t *nbl = NULL, *nbc = NULL; /* collisionType 1: contact table, built on first use */
+  uint8_t* solved = NULL;
+  uint64_t* minc = NULL;
   if (!keys0 || !keys || !ids || !starts || !ends || !X || !P || !V || !Vn || !F || !Om || !tmp ||
       !c4s || !cnt || !ccnt || !dens) {
     rc = PBD_ERR_NOMEM;
@@ -749,8 +811,21 @@ int orc_step(const pbd_params* prm, orc_io* io) {
         double m = pass_collide(&W, P, X, tmp);
         if (m < io->min_cutoff_margin) io->min_cutoff_margin = m;
       } else {
-        rc = PBD_ERR_UNSUPPORTED;
-        goto out;
+        if (!nbl) { /* the contact table of the step (cells are fixed within it) */
+          nbl = (uint32_t*)malloc((size_t)n * D.cap * 4);
+          nbc = (uint32_t*)malloc((size_t)n * 4);
+          solved = (uint8_t*)malloc((size_t)n * D.cap);
+          minc = (uint64_t*)malloc((size_t)n * 8);
+          if (!nbl || !nbc || !solved || !minc) {
+            rc = PBD_ERR_NOMEM;
+            goto out;
+          }
+          for (uint32_t s = 0; s < n; ++s) {
+            uint32_t nc;
+            nbc[s] = orc_neighbours(&D, n, keys, starts, ends, s, &nbl[(size_t)s * D.cap], &nc);
+          }
+        }
+        pass_collide_batches(&W, P, X, nbl, nbc, solved, minc);
       }
     }
     if (D.g_apply_delta) {
@@ -827,5 +902,9 @@ out:
   free(cnt);
   free(ccnt);
   free(dens);
+  free(nbl);
+  free(nbc);
+  free(solved);
+  free(minc);
   return rc;
 }

@@ -274,6 +274,27 @@ int Fluid::reset_counters() {
   return hip_check(e, "reset counters");
 }
 
+// collisionType 1 state: a solved bit per neighbour-table slot, the smallest
+// offer and the candidate partner per particle; allocated on first use.
+int Fluid::ensure_batch_buffers() {
+  const uint32_t words = (params_.max_neighbours_per_particle + 31u) / 32u;
+  if (batch_solved_ && words <= batch_words_) return PBD_OK;
+  if ((uint64_t)cap_ * words >= (1ull << 32)) return fail(PBD_ERR_INVALID, "collisionType 1: contact table too large");
+  for (void* q : {(void*)batch_solved_, (void*)batch_min_, (void*)batch_partner_}) {
+    if (!q) continue;
+    (void)hipFree(q);
+    allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), q), allocations_.end());
+  }
+  batch_solved_ = nullptr;
+  batch_min_ = nullptr;
+  batch_partner_ = nullptr;
+  if (dev_alloc(allocations_, &batch_solved_, (size_t)cap_ * words) || dev_alloc(allocations_, &batch_min_, cap_) ||
+      dev_alloc(allocations_, &batch_partner_, cap_))
+    return fail(PBD_ERR_NOMEM, "collisionType 1 buffers");
+  batch_words_ = words;
+  return PBD_OK;
+}
+
 // ---------------------------------------------------------------------------
 // The step.
 #define PBD_LAUNCH(stage, call)                                   \
@@ -320,6 +341,10 @@ int Fluid::enqueue_substep() {
   const uint32_t iters = params_.solver_iterations;
   int cur = 0;
   bool counted = false, updated = false;
+  const bool batched = params_.collision_type == 1 && S > 0;
+  if (batched) {
+    if (int rc = ensure_batch_buffers()) return rc;
+  }
   for (uint32_t it = 0; it < iters; ++it) {
     const bool last_it = it + 1 == iters;
     bool listed = false;  // the lambda pass left the neighbour lists of Q[cur]
@@ -336,11 +361,18 @@ int Fluid::enqueue_substep() {
     struct K {
       bool delta, collide;
       uint32_t flags;
+      bool batches = false;  // collisionType 1: one stabilization iteration of contact batches, in place
     };
     K ks[128];
     uint32_t nk = 0;
     const uint32_t ncoll = std::min<uint32_t>(S, 100);
-    if (delta_on) {
+    if (batched) {
+      // Fluid.cu:52-78 with collisionType 1: delta p computed first, the batches
+      // move p* and x in place, delta p applied last
+      if (delta_on) ks[nk++] = {true, false, DF_WRITE_DP};
+      for (uint32_t c = 0; c < ncoll; ++c) ks[nk++] = {false, false, 0u, true};
+      if (apply_on) ks[nk++] = {false, false, DF_APPLY};
+    } else if (delta_on) {
       uint32_t fl = 0;
       if (apply_on && ncoll <= 1) fl |= DF_APPLY;
       if (ncoll >= 2 || !apply_on || last_it) fl |= DF_WRITE_DP;
@@ -351,6 +383,12 @@ int Fluid::enqueue_substep() {
       if (ncoll == 0 && apply_on) ks[nk++] = {false, false, DF_APPLY};
     }
     for (uint32_t k = 0; k < nk; ++k) {
+      if (ks[k].batches) {
+        PBD_LAUNCH(ST_COLLIDE, launch_collide_batches(st, tpb, P, B_, B_.Q[cur], B_.QX[cur], keys,
+                                                      params_.max_collision_batches, batch_solved_, batch_words_,
+                                                      batch_min_, batch_partner_));
+        continue;
+      }
       uint32_t fl = ks[k].flags;
       if (last_it && k + 1 == nk && update_on) {
         fl |= DF_UPDATE;
@@ -397,10 +435,6 @@ int Fluid::enqueue_substep() {
 void Fluid::compute() {
   if (status_ != PBD_OK) return;
   if (n_ == 0) return;  // Fluid.cu:19
-  if (params_.collision_type == 1) {
-    fail(PBD_ERR_UNSUPPORTED, "collisionType 1 is not available in this build");
-    return;
-  }
   if (!dev_valid_) {
     const char* why = nullptr;
     if (derive_device_params(params_, n_, dev_, &why) != PBD_OK) {
@@ -424,7 +458,6 @@ int Fluid::step(uint32_t frames, bool graph) {
       }
     return PBD_OK;
   }
-  if (params_.collision_type == 1) return fail(PBD_ERR_UNSUPPORTED, "collisionType 1 is not available in this build");
   if (!graph_valid_) {
     if (!dev_valid_) {
       const char* why = nullptr;

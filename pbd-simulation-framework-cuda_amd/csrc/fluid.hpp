@@ -72,6 +72,7 @@ class Fluid : public Cuda_Computable {
   int fail(int code, const std::string& msg);
   int hip_check(hipError_t e, const char* what);
   int alloc_table();
+  int ensure_batch_buffers();
   int enqueue_substep();
   void stage_begin(int stage);
   void stage_end(int stage);
@@ -94,6 +95,10 @@ class Fluid : public Cuda_Computable {
   uint32_t* tile_count_ = nullptr;
   uint32_t* tile_flag_ = nullptr;    // per Morton block: occupied this step
   uint32_t* tile_bcount_ = nullptr;  // compaction block counts
+  uint32_t* batch_solved_ = nullptr;   // collisionType 1 (allocated on first use)
+  unsigned long long* batch_min_ = nullptr;
+  uint32_t* batch_partner_ = nullptr;
+  uint32_t batch_words_ = 0;
 
   // graph of one substep
   hipGraph_t graph_ = nullptr;

@@ -192,6 +192,92 @@ hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n) 
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// collisionType 1: contact batches (Collision.h:291-464, Fluid.cu:62-71).  The
+// contacts of particle s are its neighbour-table row, the stencil walk of
+// visit_neighbours (contacts, then shells, capped); contact c = s * cap + k.
+// A batch gives every particle to the smallest unsolved contact index at it
+// (the reference's in-block claim order, applied across blocks: its cross-block
+// claims race).  Contacts holding both particles form a matching: each is
+// marked solved and pushes its pair apart in place.
+//
+// Pass A: every unsolved contact offers its index to both particles; the row's
+// first unsolved contact is remembered as the particle's own candidate.
+__global__ void k_batch_claim(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
+                              const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+                              const uint32_t* __restrict__ solved, uint32_t words,
+                              unsigned long long* __restrict__ minc, uint32_t* __restrict__ partner) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const float4 qi = Q[s];
+  const CellFrame f = make_frame(P, keys[s]);
+  const uint32_t* sv = solved + (size_t)s * words;
+  uint32_t k = 0, nc = 0;
+  bool first = true;
+  visit_neighbours(P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0, (P.gates & G_FIND_NEIGHBOURS) != 0,
+                   cs, ce, nc, [&](uint32_t j, float, float, float, bool) {
+                     const uint32_t kk = k++;
+                     if ((sv[kk >> 5] >> (kk & 31u)) & 1u) return;
+                     const unsigned long long c = (unsigned long long)s * P.cap + kk;
+                     if (first) {
+                       first = false;
+                       partner[s] = j;
+                       atomicMin(&minc[s], c);
+                     }
+                     atomicMin(&minc[j], c);
+                   });
+}
+
+// Pass B: a particle whose smallest offer is its own candidate, and that is
+// also the smallest offer at the partner, solves that contact.
+__global__ void k_batch_solve(DevParams P, float4* __restrict__ Q, float4* __restrict__ QX,
+                              const uint32_t* __restrict__ keys, uint32_t* __restrict__ solved, uint32_t words,
+                              const unsigned long long* __restrict__ minc, const uint32_t* __restrict__ partner) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const unsigned long long c = minc[s];
+  if (c == ~0ull || c / P.cap != s) return;
+  const uint32_t j = partner[s];
+  if (minc[j] != c) return;
+  const uint32_t k = (uint32_t)(c % P.cap);
+  solved[(size_t)s * words + (k >> 5)] |= 1u << (k & 31u);  // row s is only written by its own thread
+  const uint32_t ks = keys[s], kj = keys[j];
+  const float4 qs = Q[s], qj = Q[j];
+  // p_j - p_s with each position in its own cell frame
+  const float dx = (qj.x - qs.x) + (cell_corner(P.origin[0], P.cell, compact10(kj >> 2)) -
+                                    cell_corner(P.origin[0], P.cell, compact10(ks >> 2)));
+  const float dy = (qj.y - qs.y) + (cell_corner(P.origin[1], P.cell, compact10(kj >> 1)) -
+                                    cell_corner(P.origin[1], P.cell, compact10(ks >> 1)));
+  const float dz = (qj.z - qs.z) + (cell_corner(P.origin[2], P.cell, compact10(kj)) -
+                                    cell_corner(P.origin[2], P.cell, compact10(ks)));
+  const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+  const float half = (P.diameter - len) * 0.5f;
+  if (half > 0.0f && len > 0.00001f) {
+    // both move half the overlap times (1 - stiffness); coll_scale = -(1 - stiffness) / 2
+    const float m = -2.0f * P.coll_scale * half / len;
+    Q[s] = make_float4(qs.x - m * dx, qs.y - m * dy, qs.z - m * dz, 0.0f);  // p* is written back with w = 0
+    Q[j] = make_float4(qj.x + m * dx, qj.y + m * dy, qj.z + m * dz, 0.0f);
+    const float4 xs = QX[s], xj = QX[j];
+    QX[s] = make_float4(xs.x - m * dx, xs.y - m * dy, xs.z - m * dz, xs.w);
+    QX[j] = make_float4(xj.x + m * dx, xj.y + m * dy, xj.z + m * dz, xj.w);
+  }
+}
+
+hipError_t launch_collide_batches(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,
+                                  float4* Q, float4* QX, const uint32_t* keys, uint32_t batches, uint32_t* solved,
+                                  uint32_t words, unsigned long long* minc, uint32_t* partner) {
+  const dim3 g(blocks_for(P.n, tpb)), b(tpb);
+  hipError_t e = launch_fill_u32(st, solved, 0u, P.n * words);
+  for (uint32_t i = 0; i < batches && e == hipSuccess; ++i) {
+    e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(minc), 0xFFFFFFFFu, 2u * P.n);
+    if (e != hipSuccess) break;
+    hipLaunchKernelGGL(k_batch_claim, g, b, 0, st, P, Q, keys, B.cstart, B.cend, solved, words, minc, partner);
+    hipLaunchKernelGGL(k_batch_solve, g, b, 0, st, P, Q, QX, keys, solved, words, minc, partner);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
 hipError_t launch_predict_keys(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B) {
   hipLaunchKernelGGL(k_predict_keys, dim3(blocks_for(P.n, tpb)), dim3(tpb), 0, st, P, B.X, B.V, B.F, B.P, B.keys_in);
   return hipGetLastError();

@@ -80,6 +80,11 @@ hipError_t launch_omega(hipStream_t st, uint32_t tpb, const DevParams& P, const 
                         const uint32_t* keys_s);
 hipError_t launch_final(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, uint32_t flags,
                         const float4* Q, const float4* QX, const uint32_t* keys_s);
+// collisionType 1: one stabilization iteration of `batches` contact batches,
+// in place on Q / QX (solved: n x words bit rows, minc: n, partner: n).
+hipError_t launch_collide_batches(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,
+                                  float4* Q, float4* QX, const uint32_t* keys, uint32_t batches, uint32_t* solved,
+                                  uint32_t words, unsigned long long* minc, uint32_t* partner);
 hipError_t launch_check_finite(hipStream_t st, uint32_t n, const float4* X, const float4* V, uint32_t* bad);
 
 // ---- LDS cell-block tiles (tiles.hip), textbook variant, 27-cell stencil ----

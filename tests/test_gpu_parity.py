@@ -214,6 +214,54 @@ def test_gpu_neighbour_cap(F, O):
     assert g["neighbour_counters"].max() == 40
 
 
+def test_gpu_collision_batches_line_of_three(F, O):
+    # collisionType 1 on three particles in a line (the hand-derived batch sequence of
+    # tests/test_oracle.py: (0,1), (0,2) apart, (1,0), (1,2))
+    from test_oracle import _batch_params, _push
+    x = [float(np.float32(v)) for v in (30.2, 30.8, 31.4)]
+    a, b, c = x
+    a, b = _push(a, b)
+    a, b = _push(a, b)
+    b, c = _push(b, c)
+    pos = np.zeros((3, 4), np.float32)
+    pos[:, 0] = x
+    pos[:, 1:3] = 30.5
+    p = _batch_params(F, 4)
+    flush("[batches] line of three, 4 batches")
+    ctx = make_ctx(F, p, pos)
+    ctx.compute()
+    got = ctx.download("predicted")
+    gx = ctx.download("positions")
+    ctx.close()
+    assert np.allclose(got[:, 0], [a, b, c], rtol=0, atol=2e-5), got[:, 0]
+    assert np.allclose(gx[:, 0], [a, b, c], rtol=0, atol=2e-5)
+    assert np.allclose(got[:, 1:3], 30.5)
+
+
+@pytest.mark.parametrize("variant", ["textbook", "reference"])
+def test_gpu_step_collision_batches(F, O, variant):
+    # collisionType 1: delta p, then maxCollisionBatches contact batches per stabilization
+    # iteration, then apply (Fluid.cu:52-78)
+    if variant == "textbook":
+        p = small_dam(F, 10, collision_type=1, max_collision_batches=16, stabilization_iterations=2)
+        st = developed_state(F, O, small_dam(F, 10), 5)
+        compare_one_step(F, O, p, st, "textbook 10^3, collisionType 1", conditioned=True)
+    else:
+        p = F.load_params(F.config_path("reference"))
+        p.box_width = p.box_height = p.box_depth = 8
+        p.box_jitter = 0.05
+        p.collision_type = 1
+        p.max_collision_batches = 16
+        for seed in range(1, 8):
+            p.box_seed_lo = seed
+            st = developed_state(F, O, p, 3)
+            probe = O.step(p, copy_state(O, st))
+            if probe["min_cutoff_margin"] > 2e-5:
+                break
+        compare_one_step(F, O, p, st, f"reference 8^3, collisionType 1 (seed {seed})", check_margin=2e-5,
+                         conditioned=True)
+
+
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("case", ["one", "two", "tile_plus_one", "all_in_one_cell", "wide_keys"])
 def test_gpu_integer_stages_edge_cases(F, O, case):

@@ -333,6 +333,64 @@ def test_collision_two_particles(F, O):
     assert out["contact_count"].tolist() == [1, 1]
 
 
+def _batch_params(F, batches):
+    p = F.default_params()  # diameter 1, stiffness 0.3, grid cell 1
+    p.gravity = 0.0
+    p.active_compute_lambda = p.active_compute_delta_positions = p.active_apply_delta_positions = 0
+    p.active_compute_omegas = p.active_compute_vorticity = p.active_compute_viscosity = 0
+    p.active_update_positions = 0
+    p.solver_iterations = p.stabilization_iterations = 1
+    p.collision_type = 1
+    p.max_collision_batches = batches
+    return p
+
+
+def _push(a, b, stiff=float(np.float32(0.3))):
+    """One contact of collisionType 1 on a line (Collision.h:385-439): both move half the overlap * (1 - s)."""
+    half = (1.0 - (b - a)) / 2.0
+    m = half * (1.0 - stiff) if half > 0 else 0.0
+    return a - m, b + m
+
+
+def test_collision_batches_line_of_three(F, O):
+    # particles 0, 1 in cell x = 30 and 2 in cell x = 31 (sorted order 0, 1, 2).  Contact rows:
+    # 0: [1, 2], 1: [0, 2], 2: [0, 1]; contact index c = row * cap + slot.  Each batch solves the
+    # contacts whose index is the smallest unsolved one at both particles:
+    #   batch 1: (0,1); 2: (0,2), apart, no push; 3: (1,0); 4: (1,2); 5: (2,0), apart; 6: (2,1)
+    x = [float(np.float32(v)) for v in (30.2, 30.8, 31.4)]
+    pos = np.zeros((3, 4), np.float32)
+    pos[:, 0] = x
+    pos[:, 1:3] = 30.5
+    expect = {}
+    a, b, c = x
+    a, b = _push(a, b)                     # batch 1
+    expect[1] = expect[2] = (a, b, c)      # batch 2: (0,2) is 1.34 apart
+    a, b = _push(a, b)                     # batch 3
+    expect[3] = (a, b, c)
+    b, c = _push(b, c)                     # batch 4
+    expect[4] = expect[5] = (a, b, c)      # batch 5: (2,0) apart
+    b, c = _push(b, c)                     # batch 6
+    expect[6] = expect[16] = (a, b, c)     # nothing left unsolved
+    for batches, e in expect.items():
+        out = O.step(_batch_params(F, batches), O.State(pos))
+        assert out["ids"].tolist() == [0, 1, 2]
+        for k in range(3):
+            assert out["p4"][k, 0] == pytest.approx(e[k], rel=1e-14), (batches, k)
+            assert out["x4"][k, 0] == pytest.approx(e[k], rel=1e-14), (batches, k)
+
+
+def test_collision_batches_conserve_centre(F, O):
+    # every solved contact moves its two particles by opposite amounts
+    rng = np.random.default_rng(5)
+    pos = np.zeros((40, 4), np.float32)
+    pos[:, :3] = 30.0 + rng.random((40, 3)).astype(np.float32) * 2.5
+    out = O.step(_batch_params(F, 16), O.State(pos))
+    before = pos[out["ids"], :3].astype(np.float64).mean(axis=0)
+    after = out["p4"][:, :3].mean(axis=0)
+    assert np.allclose(after, before, rtol=0, atol=1e-12)
+    assert np.abs(out["p4"][:, :3] - pos[out["ids"], :3]).max() > 0.01  # overlapping: particles moved
+
+
 def test_free_fall_semi_implicit(F, O):
     # an isolated particle: predict v += g dt, p = x + v dt (FP32, Forces.h:81-87),
     # update v = (p - x)/dt, x = p (Update.h:24-29); no neighbours so no solver term
