@@ -270,6 +270,34 @@ int pbd_get_counters(pbd_ctx* ctx, uint64_t* values, uint32_t cap, uint32_t* cou
 int pbd_reset_counters(pbd_ctx* ctx);
 /* Last error message of this context (or of the last failed pbd_create when ctx is NULL). */
 const char* pbd_last_error(pbd_ctx* ctx);
+
+/* ---------------------------------------------------------------------------
+ * Batched scenes (no counterpart in the reference, which steps one scene per
+ * Fluid::compute, Fluid.cu:14-97).  n scenes share one context and are
+ * stepped together: one launch per stage (one graph per substep) for all of
+ * them.  Scenes may differ in enclosure (bounds), enclosure damping,
+ * relaxation and spawn box; all other parameters must equal scene 0's
+ * (PBD_ERR_INVALID otherwise).  Each scene spawns its box at creation; the
+ * particle count of a batch is fixed.  Scene s lives in the batch's frame
+ * translated by offset_s (pbd_batch_scene); downloads in PBD_FRAME_SCENE
+ * subtract it from positions. */
+typedef struct pbd_batch pbd_batch;
+enum { PBD_FRAME_SCENE = 0, PBD_FRAME_BATCH = 1 };
+int pbd_batch_create(const pbd_params* scenes, uint32_t n_scenes, pbd_batch** out);
+/* frames x substeps substeps of every scene, one graph launch each */
+int pbd_batch_step(pbd_batch* b, uint32_t frames);
+/* one substep of every scene, launched kernel by kernel */
+int pbd_batch_compute(pbd_batch* b);
+int pbd_batch_sync(pbd_batch* b);
+int pbd_batch_num_scenes(pbd_batch* b, uint32_t* n);
+/* particle count, offset (batch frame = scene frame + offset) and the scene's
+ * parameters in the batch's frame (grid and enclosure as the kernels see them) */
+int pbd_batch_scene(pbd_batch* b, uint32_t scene, uint32_t* count, float offset[3], pbd_params* params);
+/* per-particle buffer (PBD_BUF_*, not the cell tables) of one scene in sorted
+ * order; bytes = count x element size */
+int pbd_batch_download(pbd_batch* b, uint32_t scene, int which, int frame, void* dst, size_t bytes);
+const char* pbd_batch_last_error(pbd_batch* b);
+void pbd_batch_destroy(pbd_batch* b);
 /* Free the context (cudaCleanupKernels, kernel/Initialize.h:136-140). */
 void pbd_destroy(pbd_ctx* ctx);
 /* ABI version (PBD_ABI_VERSION) and the gfx target the kernels were built for. */

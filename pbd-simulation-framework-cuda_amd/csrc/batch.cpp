@@ -1,0 +1,277 @@
+// batch.cpp - many scenes stepped by one context (pbd_batch_*).
+//
+// The reference steps one scene per Fluid::compute (src/app/simulation/Fluid.cu:14-97).
+// A batch places n scenes in one grid: scene s occupies the cubic slot of
+// 2^m cells per axis whose Morton index is s, with its enclosure translated
+// to start g = kernelWidthNeighbours + 1 cells inside the slot. The slot is
+// Morton-aligned, so the cell ids of a scene are one contiguous range, the
+// sort keeps every scene contiguous and in scene order, and no stencil reaches
+// across the empty cells between scenes. One set of launches (and one graph)
+// then steps all scenes at once with every workgroup of the GPU busy. Scenes
+// differ in enclosure, damping, relaxation and spawn box; the solver shape
+// (kernels, iterations, gates, time step, cell size) is shared. Those per-scene
+// constants reach the kernels through a device table (SceneParams) indexed by
+// the scene of a particle's cell.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "fluid.hpp"
+#include "params.hpp"
+#include "pbd/pbd.h"
+
+struct pbd_batch {
+  pbd::Fluid fluid;
+  std::vector<pbd_params> world;   // each scene in the batch's frame
+  std::vector<float> offset;       // 3 per scene: batch frame = scene frame + offset
+  std::vector<uint32_t> count, first;
+  pbd::SceneParams* table = nullptr;
+  std::string error;
+  ~pbd_batch() {
+    if (table) {
+      (void)fluid.sync();  // queued steps read the table
+      (void)hipFree(table);
+    }
+  }
+};
+
+namespace {
+
+thread_local std::string g_batch_error;
+
+// The fields a scene may set on its own; all others must equal scene 0's.
+void mask_scene_fields(pbd_params& q, const pbd_params& ref) {
+  q.texture_width = ref.texture_width;
+  q.relaxation = ref.relaxation;
+  q.enclosure_position_damping = ref.enclosure_position_damping;
+  q.enclosure_velocity_damping = ref.enclosure_velocity_damping;
+  for (int a = 0; a < 3; ++a) {
+    q.bounds_min[a] = ref.bounds_min[a];
+    q.bounds_max[a] = ref.bounds_max[a];
+    q.box_origin[a] = ref.box_origin[a];
+    q.grid_origin[a] = ref.grid_origin[a];
+    q.grid_dims[a] = ref.grid_dims[a];
+  }
+  q.box_width = ref.box_width;
+  q.box_height = ref.box_height;
+  q.box_depth = ref.box_depth;
+  q.box_spawn_in_the_beginning = ref.box_spawn_in_the_beginning;
+  q.box_jitter = ref.box_jitter;
+  q.box_seed_lo = ref.box_seed_lo;
+  q.box_seed_hi = ref.box_seed_hi;
+}
+
+int fail(pbd_batch* b, int rc, const std::string& msg) {
+  if (b) b->error = msg;
+  g_batch_error = msg;
+  return rc;
+}
+
+int build(pbd_batch* b, const pbd_params* scenes, uint32_t n) {
+  using namespace pbd;
+  const pbd_params& p0 = scenes[0];
+  DevParams d0;
+  const char* why = nullptr;
+  for (uint32_t s = 0; s < n; ++s) {
+    if (derive_device_params(scenes[s], 0, d0, &why) != PBD_OK)
+      return fail(b, PBD_ERR_INVALID, "scene " + std::to_string(s) + ": " + (why ? why : "invalid parameters"));
+    pbd_params q = scenes[s];
+    mask_scene_fields(q, p0);
+    if (std::memcmp(&q, &p0, sizeof q) != 0)
+      return fail(b, PBD_ERR_INVALID,
+                  "scene " + std::to_string(s) +
+                      " differs from scene 0 in a shared parameter (scenes may differ only in enclosure, damping, "
+                      "relaxation and spawn box)");
+  }
+  if (derive_device_params(p0, 0, d0, &why) != PBD_OK) return fail(b, PBD_ERR_INVALID, why ? why : "scene 0");
+  const double cell = d0.cell;
+  const uint32_t gap = (uint32_t)std::max(1, d0.kw) + 1u;
+  // slot size: the largest enclosure plus the gap on both sides, in cells, rounded to a power of two
+  uint32_t m = 0;
+  for (uint32_t s = 0; s < n; ++s)
+    for (int a = 0; a < 3; ++a) {
+      const double ext = (double)scenes[s].bounds_max[a] - (double)scenes[s].bounds_min[a];
+      if (!(ext > 0.0)) return fail(b, PBD_ERR_INVALID, "scene " + std::to_string(s) + ": empty enclosure");
+      const uint64_t need = (uint64_t)std::ceil(ext / cell) + 2ull * gap;
+      while ((1ull << m) < need) ++m;
+    }
+  uint32_t smax[3] = {0, 0, 0};
+  for (uint32_t s = 0; s < n; ++s) {
+    const uint32_t sc[3] = {compact10(s >> 2), compact10(s >> 1), compact10(s)};
+    for (int a = 0; a < 3; ++a) smax[a] = std::max(smax[a], sc[a]);
+  }
+  uint32_t dims[3];
+  for (int a = 0; a < 3; ++a) {
+    const uint64_t d = (uint64_t)(smax[a] + 1) << m;
+    if (d > 1024) return fail(b, PBD_ERR_INVALID, "batch: the scenes do not fit in a 1024^3-cell grid");
+    dims[a] = (uint32_t)d;
+  }
+  if (n > (1u << 30) >> (3 * m)) return fail(b, PBD_ERR_INVALID, "batch: too many scenes");
+  // each scene in the batch's frame
+  b->world.resize(n);
+  b->offset.resize(3 * (size_t)n);
+  b->count.assign(n, 0);
+  b->first.assign(n, 0);
+  uint64_t total = 0;
+  for (uint32_t s = 0; s < n; ++s) {
+    pbd_params w = scenes[s];
+    const uint32_t sc[3] = {compact10(s >> 2), compact10(s >> 1), compact10(s)};
+    for (int a = 0; a < 3; ++a) {
+      const float off = (float)((double)((sc[a] << m) + gap) * cell - (double)scenes[s].bounds_min[a]);
+      b->offset[3 * s + a] = off;
+      w.bounds_min[a] = scenes[s].bounds_min[a] + off;
+      w.bounds_max[a] = scenes[s].bounds_max[a] + off;
+      w.box_origin[a] = scenes[s].box_origin[a] + off;
+      w.grid_origin[a] = 0.0f;
+      w.grid_dims[a] = dims[a];
+    }
+    w.grid_cell_size = (float)cell;
+    b->world[s] = w;
+    if (w.box_spawn_in_the_beginning) total += (uint64_t)w.box_width * w.box_height * w.box_depth;
+  }
+  if (total == 0) return fail(b, PBD_ERR_INVALID, "batch: no scene spawns particles");
+  if (total >= (1ull << 30)) return fail(b, PBD_ERR_CAPACITY, "batch: more than 2^30 particles");
+  // per-scene constants
+  std::vector<SceneParams> host(n);
+  for (uint32_t s = 0; s < n; ++s) {
+    DevParams d;
+    if (derive_device_params(b->world[s], 0, d, &why) != PBD_OK)
+      return fail(b, PBD_ERR_INVALID, "scene " + std::to_string(s) + ": " + (why ? why : "invalid in the batch"));
+    SceneParams& t = host[s];
+    for (int a = 0; a < 3; ++a) {
+      t.bmin[a] = d.bmin[a];
+      t.bmax[a] = d.bmax[a];
+      t.lo[a] = d.lo[a];
+      t.hi[a] = d.hi[a];
+    }
+    t.vdamp = d.vdamp;
+    t.pdamp = d.pdamp;
+    t.eps = d.eps;
+    t.pad = 0.0f;
+  }
+  // the context: scene 0's solver over the whole grid, no spawn of its own
+  pbd_params all = b->world[0];
+  for (int a = 0; a < 3; ++a) {
+    all.bounds_min[a] = 0.0f;
+    all.bounds_max[a] = (float)(dims[a] * cell);
+  }
+  all.box_spawn_in_the_beginning = 0;
+  if (int rc = b->fluid.init(all, (uint32_t)total)) return fail(b, rc, b->fluid.error());
+  if (hipMalloc(&b->table, sizeof(SceneParams) * n) != hipSuccess) return fail(b, PBD_ERR_NOMEM, "scene table");
+  if (hipMemcpy(b->table, host.data(), sizeof(SceneParams) * n, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(b, PBD_ERR_HIP, "scene table upload");
+  b->fluid.set_scenes(b->table, n, m);
+  // spawn every scene's box, in scene order (= slot order = sorted order)
+  uint32_t at = 0;
+  for (uint32_t s = 0; s < n; ++s) {
+    const pbd_params& w = b->world[s];
+    b->first[s] = at;
+    if (!w.box_spawn_in_the_beginning) continue;
+    const uint32_t cnt = w.box_width * w.box_height * w.box_depth;
+    std::vector<float> pos((size_t)cnt * 4), vel((size_t)cnt * 4), col((size_t)cnt * 4);
+    uint32_t got = 0;
+    if (int rc = pbd_generate_box(&w, pos.data(), vel.data(), col.data(), cnt, &got)) return fail(b, rc, "spawn");
+    if (int rc = b->fluid.add_particles(got, pos.data(), vel.data(), col.data())) return fail(b, rc, b->fluid.error());
+    b->count[s] = got;
+    at += got;
+  }
+  return PBD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pbd_batch_create(const pbd_params* scenes, uint32_t n_scenes, pbd_batch** out) {
+  if (!out) return PBD_ERR_INVALID;
+  *out = nullptr;
+  if (!scenes || n_scenes == 0) return fail(nullptr, PBD_ERR_INVALID, "pbd_batch_create: no scenes");
+  try {
+    pbd_batch* b = new pbd_batch();
+    const int rc = build(b, scenes, n_scenes);
+    if (rc != PBD_OK) {
+      g_batch_error = b->error;
+      delete b;
+      return rc;
+    }
+    *out = b;
+    return PBD_OK;
+  } catch (const std::bad_alloc&) {
+    return fail(nullptr, PBD_ERR_NOMEM, "pbd_batch_create: out of host memory");
+  }
+}
+
+int pbd_batch_step(pbd_batch* b, uint32_t frames) {
+  if (!b) return PBD_ERR_INVALID;
+  b->fluid.clear_status();
+  const int rc = b->fluid.step(frames, true);
+  if (rc != PBD_OK) b->error = b->fluid.error();
+  return rc;
+}
+
+int pbd_batch_compute(pbd_batch* b) {
+  if (!b) return PBD_ERR_INVALID;
+  b->fluid.clear_status();
+  b->fluid.compute();
+  const int rc = b->fluid.status();
+  if (rc != PBD_OK) b->error = b->fluid.error();
+  return rc;
+}
+
+int pbd_batch_sync(pbd_batch* b) {
+  if (!b) return PBD_ERR_INVALID;
+  const int rc = b->fluid.sync();
+  if (rc != PBD_OK) b->error = b->fluid.error();
+  return rc;
+}
+
+int pbd_batch_num_scenes(pbd_batch* b, uint32_t* n) {
+  if (!b || !n) return PBD_ERR_INVALID;
+  *n = (uint32_t)b->count.size();
+  return PBD_OK;
+}
+
+int pbd_batch_scene(pbd_batch* b, uint32_t scene, uint32_t* count, float offset[3], pbd_params* params) {
+  if (!b || scene >= b->count.size()) return PBD_ERR_INVALID;
+  if (count) *count = b->count[scene];
+  if (offset)
+    for (int a = 0; a < 3; ++a) offset[a] = b->offset[3 * (size_t)scene + a];
+  if (params) *params = b->world[scene];
+  return PBD_OK;
+}
+
+int pbd_batch_download(pbd_batch* b, uint32_t scene, int which, int frame, void* dst, size_t bytes) {
+  if (!b || scene >= b->count.size()) return PBD_ERR_INVALID;
+  const uint32_t cnt = b->count[scene];
+  const bool f4 = which == PBD_BUF_POSITIONS || which == PBD_BUF_PREDICTED || which == PBD_BUF_VELOCITIES ||
+                  which == PBD_BUF_COLORS || which == PBD_BUF_EXTERNAL_FORCES || which == PBD_BUF_OMEGAS ||
+                  which == PBD_BUF_DELTA_POSITIONS;
+  const size_t need = (size_t)cnt * (f4 ? 16 : 4);
+  if (bytes != need) {
+    b->error = "pbd_batch_download: bytes must be the scene's count times the element size";
+    return PBD_ERR_INVALID;
+  }
+  int rc = b->fluid.download_range(which, b->first[scene], cnt, dst);
+  if (rc != PBD_OK) {
+    b->error = b->fluid.error();
+    return rc;
+  }
+  if (frame == PBD_FRAME_SCENE && (which == PBD_BUF_POSITIONS || which == PBD_BUF_PREDICTED)) {
+    float* v = static_cast<float*>(dst);
+    const float* off = &b->offset[3 * (size_t)scene];
+    for (uint32_t i = 0; i < cnt; ++i)
+      for (int a = 0; a < 3; ++a) v[4 * (size_t)i + a] -= off[a];
+  }
+  return PBD_OK;
+}
+
+const char* pbd_batch_last_error(pbd_batch* b) { return b ? b->error.c_str() : g_batch_error.c_str(); }
+
+void pbd_batch_destroy(pbd_batch* b) { delete b; }
+
+}  // extern "C"

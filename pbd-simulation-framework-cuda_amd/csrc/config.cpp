@@ -678,6 +678,9 @@ int derive_device_params(const pbd_params& p, uint32_t n, DevParams& o, const ch
     o.list_r2 = r * r * 1.0001f;
   }
   o.e_vort = p.e_vorticity;
+  o.scenes = nullptr;  // one scene; a batched context installs its scene table
+  o.n_scenes = 1;
+  o.slot_log2 = 10;
   return PBD_OK;
 }
 

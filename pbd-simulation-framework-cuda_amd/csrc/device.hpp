@@ -76,6 +76,61 @@ __host__ __device__ __forceinline__ float cell_corner(float origin, float cell, 
 }
 
 // ---------------------------------------------------------------------------
+// The enclosure of a particle: the context's, or in a batched context that of
+// the scene whose slot holds position x.
+struct Enclosure {
+  float bmin[3], bmax[3], vdamp, pdamp;
+};
+
+__host__ __device__ __forceinline__ uint32_t scene_of_cells(const DevParams& P, uint32_t ix, uint32_t iy,
+                                                            uint32_t iz) {
+  const uint32_t s = morton3(ix >> P.slot_log2, iy >> P.slot_log2, iz >> P.slot_log2);
+  return s < P.n_scenes ? s : P.n_scenes - 1u;
+}
+
+__host__ __device__ __forceinline__ Enclosure enclosure_of(const DevParams& P, const float4& x) {
+  Enclosure e;
+  if (!P.scenes) {
+    for (int a = 0; a < 3; ++a) {
+      e.bmin[a] = P.bmin[a];
+      e.bmax[a] = P.bmax[a];
+    }
+    e.vdamp = P.vdamp;
+    e.pdamp = P.pdamp;
+    return e;
+  }
+  const uint32_t s = scene_of_cells(P, cell_coord(x.x, P.origin[0], P.inv_cell, P.dims[0]),
+                                    cell_coord(x.y, P.origin[1], P.inv_cell, P.dims[1]),
+                                    cell_coord(x.z, P.origin[2], P.inv_cell, P.dims[2]));
+  const SceneParams& S = P.scenes[s];
+  for (int a = 0; a < 3; ++a) {
+    e.bmin[a] = S.bmin[a];
+    e.bmax[a] = S.bmax[a];
+  }
+  e.vdamp = S.vdamp;
+  e.pdamp = S.pdamp;
+  return e;
+}
+
+// Relaxation (Density.relaxation) and the projection box of the particle in cell `key`.
+__host__ __device__ __forceinline__ float scene_eps(const DevParams& P, uint32_t key) {
+  return P.scenes ? P.scenes[key >> (3u * P.slot_log2)].eps : P.eps;
+}
+struct ProjBox {
+  float lo[3], hi[3];
+};
+__host__ __device__ __forceinline__ ProjBox scene_box(const DevParams& P, uint32_t key) {
+  ProjBox b;
+  const float* lo = P.scenes ? P.scenes[key >> (3u * P.slot_log2)].lo : P.lo;
+  const float* hi = P.scenes ? P.scenes[key >> (3u * P.slot_log2)].hi : P.hi;
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = lo[a];
+    b.hi[a] = hi[a];
+  }
+  return b;
+}
+
+// ---------------------------------------------------------------------------
 // applyForces + confineToBox (Forces.h:19-101) with the reference's FP32
 // operation order and no contraction:
 //   v.y += (1*g)*dt;  v += f*dt;  p = x + v*dt;  clamp p per axis to the
@@ -83,6 +138,7 @@ __host__ __device__ __forceinline__ float cell_corner(float origin, float cell, 
 //   damping); if anything clamped x += pdamp*(p - x);  p.w = 1.
 __host__ __device__ __forceinline__ void predict(const DevParams& P, float4& x, float4& v, const float4 f, float4& p) {
 #pragma clang fp contract(off)
+  const Enclosure E = enclosure_of(P, x);
   const float dt = P.dt;
   const float gdt = (1.0f * P.gravity) * dt;
   v.y = v.y + gdt;
@@ -95,38 +151,38 @@ __host__ __device__ __forceinline__ void predict(const DevParams& P, float4& x, 
   p.z = x.z + v.z * dt;
   p.w = x.w + v.w * dt;
   bool upd = false;
-  if (p.x < P.bmin[0]) {
-    v.x = P.vdamp * v.x;
-    p.x = P.bmin[0] + 0.001f;
+  if (p.x < E.bmin[0]) {
+    v.x = E.vdamp * v.x;
+    p.x = E.bmin[0] + 0.001f;
     upd = true;
-  } else if (p.x > P.bmax[0]) {
-    v.x = P.vdamp * v.x;
-    p.x = P.bmax[0] - 0.001f;
-    upd = true;
-  }
-  if (p.y < P.bmin[1]) {
-    v.y = P.vdamp * v.y;
-    p.y = P.bmin[1] + 0.001f;
-    upd = true;
-  } else if (p.y > P.bmax[1]) {
-    v.y = P.vdamp * v.y;
-    p.y = P.bmax[1] - 0.001f;
+  } else if (p.x > E.bmax[0]) {
+    v.x = E.vdamp * v.x;
+    p.x = E.bmax[0] - 0.001f;
     upd = true;
   }
-  if (p.z < P.bmin[2]) {
-    v.z = P.vdamp * v.z;
-    p.z = P.bmin[2] + 0.001f;
+  if (p.y < E.bmin[1]) {
+    v.y = E.vdamp * v.y;
+    p.y = E.bmin[1] + 0.001f;
     upd = true;
-  } else if (p.z > P.bmax[2]) {
-    v.z = P.vdamp * v.z;
-    p.z = P.bmax[2] - 0.001f;
+  } else if (p.y > E.bmax[1]) {
+    v.y = E.vdamp * v.y;
+    p.y = E.bmax[1] - 0.001f;
+    upd = true;
+  }
+  if (p.z < E.bmin[2]) {
+    v.z = E.vdamp * v.z;
+    p.z = E.bmin[2] + 0.001f;
+    upd = true;
+  } else if (p.z > E.bmax[2]) {
+    v.z = E.vdamp * v.z;
+    p.z = E.bmax[2] - 0.001f;
     upd = true;
   }
   if (upd) {
-    x.x = x.x + P.pdamp * (p.x - x.x);
-    x.y = x.y + P.pdamp * (p.y - x.y);
-    x.z = x.z + P.pdamp * (p.z - x.z);
-    x.w = x.w + P.pdamp * (p.w - x.w);
+    x.x = x.x + E.pdamp * (p.x - x.x);
+    x.y = x.y + E.pdamp * (p.y - x.y);
+    x.z = x.z + E.pdamp * (p.z - x.z);
+    x.w = x.w + E.pdamp * (p.w - x.w);
   }
   p.w = 1.0f;
 }

@@ -274,6 +274,21 @@ int Fluid::reset_counters() {
   return hip_check(e, "reset counters");
 }
 
+void Fluid::set_scenes(const SceneParams* table, uint32_t n, uint32_t slot_log2) {
+  scene_table_ = table;
+  scene_count_ = n;
+  slot_log2_ = slot_log2;
+  dev_valid_ = false;
+  invalidate_graph();
+}
+
+void Fluid::apply_scenes() {
+  if (!scene_table_) return;
+  dev_.scenes = scene_table_;
+  dev_.n_scenes = scene_count_;
+  dev_.slot_log2 = slot_log2_;
+}
+
 // collisionType 1 state: a solved bit per neighbour-table slot, the smallest
 // offer and the candidate partner per particle; allocated on first use.
 int Fluid::ensure_batch_buffers() {
@@ -441,6 +456,7 @@ void Fluid::compute() {
       fail(PBD_ERR_INVALID, why);
       return;
     }
+    apply_scenes();
     dev_valid_ = true;
   }
   enqueue_substep();
@@ -462,6 +478,7 @@ int Fluid::step(uint32_t frames, bool graph) {
     if (!dev_valid_) {
       const char* why = nullptr;
       if (derive_device_params(params_, n_, dev_, &why) != PBD_OK) return fail(PBD_ERR_INVALID, why);
+      apply_scenes();
       dev_valid_ = true;
     }
     if (hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
@@ -486,6 +503,7 @@ int Fluid::step(uint32_t frames, bool graph) {
 }
 
 int Fluid::sync() {
+  if (!stream_ || !sort_.err) return status_;  // not initialised: nothing queued
   if (hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize")) return status_;
   if (timing_ || !pending_.empty()) collect_timings();
   uint32_t err = 0;
@@ -559,6 +577,33 @@ int Fluid::download(int which, void* dst, size_t bytes) {
   }
   if (bytes == 0) return PBD_OK;
   return hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "download");
+}
+
+int Fluid::download_range(int which, uint32_t first, uint32_t count, void* dst) {
+  size_t esz = 0;
+  const void* src = nullptr;
+  switch (which) {
+    case PBD_BUF_POSITIONS: src = B_.X; esz = 16; break;
+    case PBD_BUF_PREDICTED: src = B_.P; esz = 16; break;
+    case PBD_BUF_VELOCITIES: src = B_.V; esz = 16; break;
+    case PBD_BUF_COLORS: src = B_.C; esz = 16; break;
+    case PBD_BUF_EXTERNAL_FORCES: src = B_.F; esz = 16; break;
+    case PBD_BUF_DENSITIES: src = B_.DENS; esz = 4; break;
+    case PBD_BUF_LAMBDAS: src = B_.LAM; esz = 4; break;
+    case PBD_BUF_NEIGHBOUR_COUNTERS: src = B_.NCNT; esz = 4; break;
+    case PBD_BUF_CONTACT_COUNTERS: src = B_.CCNT; esz = 4; break;
+    case PBD_BUF_OMEGAS: src = B_.OM; esz = 16; break;
+    case PBD_BUF_DELTA_POSITIONS: src = B_.DP; esz = 16; break;
+    case PBD_BUF_CELL_IDS: src = keys_sorted_; esz = 4; break;
+    case PBD_BUF_PARTICLE_IDS: src = ids_sorted_; esz = 4; break;
+    default: return fail(PBD_ERR_INVALID, "download: not a per-particle buffer");
+  }
+  if ((uint64_t)first + count > n_ || (count && !dst)) return fail(PBD_ERR_INVALID, "download: range exceeds particle count");
+  if (int rc = sync()) return rc;
+  if (count == 0) return PBD_OK;
+  return hip_check(hipMemcpy(dst, static_cast<const char*>(src) + (size_t)first * esz, (size_t)count * esz,
+                             hipMemcpyDeviceToHost),
+                   "download");
 }
 
 int Fluid::upload(int which, const void* src, size_t bytes) {

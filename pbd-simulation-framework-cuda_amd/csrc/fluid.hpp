@@ -50,6 +50,8 @@ class Fluid : public Cuda_Computable {
   uint32_t capacity() const { return cap_; }
 
   int download(int which, void* dst, size_t bytes);
+  // count elements of a per-particle buffer from index first (sorted order)
+  int download_range(int which, uint32_t first, uint32_t count, void* dst);
   int upload(int which, const void* src, size_t bytes);
   int buffer_bytes(int which, size_t* bytes) const;
   int check_finite(uint32_t* bad);
@@ -60,6 +62,11 @@ class Fluid : public Cuda_Computable {
   void reset_timings();
   int counters(uint64_t* values, uint32_t cap, uint32_t* count);
   int reset_counters();
+
+  // Batched scenes: the device table of n scene constants and the slot size
+  // (owned by the caller, alive as long as the context).
+  void set_scenes(const SceneParams* table, uint32_t n, uint32_t slot_log2);
+  hipStream_t stream() const { return stream_; }
 
   int status() const { return status_; }
   const std::string& error() const { return error_; }
@@ -73,6 +80,7 @@ class Fluid : public Cuda_Computable {
   int hip_check(hipError_t e, const char* what);
   int alloc_table();
   int ensure_batch_buffers();
+  void apply_scenes();
   int enqueue_substep();
   void stage_begin(int stage);
   void stage_end(int stage);
@@ -99,6 +107,8 @@ class Fluid : public Cuda_Computable {
   unsigned long long* batch_min_ = nullptr;
   uint32_t* batch_partner_ = nullptr;
   uint32_t batch_words_ = 0;
+  const SceneParams* scene_table_ = nullptr;  // batched scenes
+  uint32_t scene_count_ = 1, slot_log2_ = 10;
 
   // graph of one substep
   hipGraph_t graph_ = nullptr;

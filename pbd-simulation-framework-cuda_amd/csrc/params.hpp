@@ -27,6 +27,17 @@ enum GateBits : uint32_t {
   G_VISCOSITY = 1u << 9,
 };
 
+// Per-scene constants of a batched context (pbd_batch_*): several scenes share
+// one grid, each in its own cubic Morton-aligned slot of 2^slot_log2 cells per
+// axis, and may differ in enclosure, damping and relaxation.
+struct SceneParams {
+  float bmin[3], bmax[3];  // enclosure (Forces.h:19-59)
+  float lo[3], hi[3];      // enclosure inset by 0.001
+  float vdamp, pdamp;      // enclosure velocity / position damping
+  float eps;               // Density.relaxation
+  float pad;
+};
+
 struct DevParams {
   uint32_t n;          // numberOfParticles
   uint32_t cap;        // maxNeighboursPerParticle
@@ -59,6 +70,11 @@ struct DevParams {
   float e_vort;
   float inv_dt;
   float list_r2;           // neighbour-list radius^2: max(h_poly, h_spiky, diameter)^2 (padded)
+  // batched scenes: table of n_scenes entries (nullptr: one scene, the fields above);
+  // the scene of a particle is the Morton index of its slot, cell key >> 3 slot_log2
+  const SceneParams* scenes;
+  uint32_t n_scenes;
+  uint32_t slot_log2;
 };
 
 // Host-side derivation; returns PBD_OK or PBD_ERR_INVALID with a message.

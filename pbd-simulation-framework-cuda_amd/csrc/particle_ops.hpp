@@ -32,13 +32,14 @@ __device__ __forceinline__ void lambda_pair(const DevParams& P, float rx, float 
 }
 
 // lambda_i = -C_i / (sum_k |grad_k C_i|^2 + eps); W is the density of the pass.
+// eps: the relaxation of the particle's scene (scene_eps)
 template <int VAR>
-__device__ __forceinline__ float lambda_finish(const DevParams& P, const LambdaAcc& a, float& W) {
+__device__ __forceinline__ float lambda_finish(const DevParams& P, const LambdaAcc& a, float& W, float eps) {
   W = P.wc * a.A + P.w0;
   const float G2 = P.gc * P.gc * (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
   const float B2 = P.gc * P.gc * a.B;
-  if (VAR == PBD_KERNEL_TEXTBOOK) return -(W * P.inv_rho0 - 1.0f) / ((G2 + B2) * P.inv_rho0_sq + P.eps);
-  return -(W * P.inv_rho0 - 1.0f) / (B2 * P.inv_rho0 + G2 * P.inv_rho0 + P.eps);
+  if (VAR == PBD_KERNEL_TEXTBOOK) return -(W * P.inv_rho0 - 1.0f) / ((G2 + B2) * P.inv_rho0_sq + eps);
+  return -(W * P.inv_rho0 - 1.0f) / (B2 * P.inv_rho0 + G2 * P.inv_rho0 + eps);
 }
 
 // ---------------------------------------------------------------- delta p + collision
@@ -106,13 +107,14 @@ __device__ __forceinline__ void delta_pair(const DevParams& P, float li, float l
   }
 }
 
-// Apply the pass to particle s: Qo/QXo <- (q, qx) + push -+ dp (+ clamp), update.
-// Returns the new position (cell frame).
+// Apply the pass to particle s (cell `key`, corner c): Qo/QXo <- (q, qx) + push -+ dp
+// (+ projection onto the scene's enclosure), update.  Returns the new position (cell frame).
 template <int VAR, bool DELTA>
 __device__ __forceinline__ float4 delta_finish(const DevParams& P, uint32_t flags, uint32_t s, float4 qi,
                                              const float4* __restrict__ QX, const DeltaAcc& a, float cx, float cy,
-                                             float cz, float4* __restrict__ Qo, float4* __restrict__ QXo,
-                                             float4* __restrict__ DP, float4* __restrict__ VS) {
+                                             float cz, uint32_t key, float4* __restrict__ Qo,
+                                             float4* __restrict__ QXo, float4* __restrict__ DP,
+                                             float4* __restrict__ VS) {
   float dx, dy, dz;
   if (DELTA) {
     const float sc = P.gc * P.inv_rho0;
@@ -138,9 +140,10 @@ __device__ __forceinline__ float4 delta_finish(const DevParams& P, uint32_t flag
       nx += dx;
       ny += dy;
       nz += dz;
-      nx = fminf(fmaxf(nx, P.lo[0] - cx), P.hi[0] - cx);
-      ny = fminf(fmaxf(ny, P.lo[1] - cy), P.hi[1] - cy);
-      nz = fminf(fmaxf(nz, P.lo[2] - cz), P.hi[2] - cz);
+      const ProjBox b = scene_box(P, key);
+      nx = fminf(fmaxf(nx, b.lo[0] - cx), b.hi[0] - cx);
+      ny = fminf(fmaxf(ny, b.lo[1] - cy), b.hi[1] - cy);
+      nz = fminf(fmaxf(nz, b.lo[2] - cz), b.hi[2] - cz);
     }
   }
   if (flags & DF_UPDATE) {
@@ -290,7 +293,7 @@ __device__ __forceinline__ void lambda_particle(const DevParams& P, uint32_t s, 
                                           lambda_pair<VAR>(P, bx - qj.x, by - qj.y, bz - qj.z, a);
                                         });
   float W;
-  lam[s] = lambda_finish<VAR>(P, a, W);
+  lam[s] = lambda_finish<VAR>(P, a, W, scene_eps(P, keys[s]));
   if (FIRST) {
     dens[s] = W;
     ncnt[s] = cnt;
@@ -319,7 +322,7 @@ __device__ __forceinline__ float4 delta_particle(const DevParams& P, uint32_t fl
                                                        contact, a);
                      });
   }
-  return delta_finish<VAR, DELTA>(P, flags, s, qi, QX, a, f.cx, f.cy, f.cz, Qo, QXo, DP, VS);
+  return delta_finish<VAR, DELTA>(P, flags, s, qi, QX, a, f.cx, f.cy, f.cz, keys[s], Qo, QXo, DP, VS);
 }
 
 template <int VAR>

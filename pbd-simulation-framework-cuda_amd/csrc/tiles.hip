@@ -501,7 +501,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_lambda(DevParams P, TileL
       atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_LONG], 1ull);
     }
     float W;
-    lam[g] = lambda_finish<VAR>(P, a, W);
+    lam[g] = lambda_finish<VAR>(P, a, W, scene_eps(P, keys[g]));
     if (FIRST) {
       dens[g] = W;
       ncnt[g] = cnt;
@@ -559,7 +559,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_delta(DevParams P, TileLi
     const float cx = cell_corner(P.origin[0], P.cell, compact10(key >> 2));
     const float cy = cell_corner(P.origin[1], P.cell, compact10(key >> 1));
     const float cz = cell_corner(P.origin[2], P.cell, compact10(key));
-    delta_finish<VAR, true>(P, flags, g, Q[g], QX, a, cx, cy, cz, Qo, QXo, DP, VS);
+    delta_finish<VAR, true>(P, flags, g, Q[g], QX, a, cx, cy, cz, key, Qo, QXo, DP, VS);
   }
 }
 

@@ -170,6 +170,16 @@ _SIGS = {
     "pbd_get_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint32, C.POINTER(C.c_uint32)]),
     "pbd_reset_counters": (C.c_int, [C.c_void_p]),
     "pbd_last_error": (C.c_char_p, [C.c_void_p]),
+    "pbd_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "pbd_batch_step": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "pbd_batch_compute": (C.c_int, [C.c_void_p]),
+    "pbd_batch_sync": (C.c_int, [C.c_void_p]),
+    "pbd_batch_num_scenes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pbd_batch_scene": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_float),
+                                  C.c_void_p]),
+    "pbd_batch_download": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_size_t]),
+    "pbd_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "pbd_batch_destroy": (None, [C.c_void_p]),
     "pbd_destroy": (None, [C.c_void_p]),
     "pbd_version": (C.c_int, []),
     "pbd_build_target": (C.c_char_p, []),
@@ -392,3 +402,68 @@ class Context:
 
     def reset_counters(self) -> None:
         self._check(lib().pbd_reset_counters(self._ptr), "pbd_reset_counters")
+
+
+class Batch:
+    """Many scenes stepped together by one context (pbd_batch_*): one launch per
+    stage for all scenes.  Scenes may differ in enclosure, damping, relaxation
+    and spawn box; every other parameter must equal the first scene's."""
+
+    FRAME_SCENE, FRAME_BATCH = 0, 1
+
+    def __init__(self, scenes):
+        scenes = list(scenes)
+        arr = (PbdParams * len(scenes))(*scenes)
+        self._ptr = C.c_void_p(None)
+        rc = lib().pbd_batch_create(arr, len(scenes), C.byref(self._ptr))
+        if rc != PBD_OK:
+            msg = lib().pbd_batch_last_error(None)
+            raise PbdError(rc, f"pbd_batch_create: {msg.decode() if msg else ''}")
+
+    def close(self) -> None:
+        if self._ptr:
+            lib().pbd_batch_destroy(self._ptr)
+            self._ptr = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != PBD_OK:
+            msg = lib().pbd_batch_last_error(self._ptr)
+            raise PbdError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def step(self, frames: int = 1) -> None:
+        self._check(lib().pbd_batch_step(self._ptr, frames), "pbd_batch_step")
+
+    def compute(self) -> None:
+        self._check(lib().pbd_batch_compute(self._ptr), "pbd_batch_compute")
+
+    def sync(self) -> None:
+        self._check(lib().pbd_batch_sync(self._ptr), "pbd_batch_sync")
+
+    @property
+    def num_scenes(self) -> int:
+        v = C.c_uint32(0)
+        self._check(lib().pbd_batch_num_scenes(self._ptr, C.byref(v)), "pbd_batch_num_scenes")
+        return v.value
+
+    def scene(self, s: int):
+        """(particle count, offset to the batch frame, the scene's parameters in the batch frame)."""
+        cnt = C.c_uint32(0)
+        off = (C.c_float * 3)()
+        prm = PbdParams()
+        self._check(lib().pbd_batch_scene(self._ptr, s, C.byref(cnt), off, C.byref(prm)), "pbd_batch_scene")
+        return cnt.value, np.array(off[:], np.float32), prm
+
+    def download(self, s: int, which, frame: int = 0) -> np.ndarray:
+        w = BUF[which] if isinstance(which, str) else int(which)
+        cnt = self.scene(s)[0]
+        dt, comp = _BUF_DTYPE[w]
+        out = np.zeros((cnt, comp) if comp > 1 else (cnt,), dt)
+        self._check(lib().pbd_batch_download(self._ptr, s, w, frame, out.ctypes.data, out.nbytes),
+                    "pbd_batch_download")
+        return out

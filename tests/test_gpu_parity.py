@@ -262,6 +262,69 @@ def test_gpu_step_collision_batches(F, O, variant):
                          conditioned=True)
 
 
+def _batch_scenes(F, n=5):
+    """Textbook scenes that differ in enclosure, relaxation, damping and spawn box."""
+    scenes = []
+    for k in range(n):
+        p = small_dam(F, 8)
+        p.bounds_max[0] = 1.2 + 0.3 * k
+        p.bounds_max[1] = 1.0 + 0.1 * k
+        p.bounds_max[2] = 0.8 + 0.2 * (k % 2)
+        p.box_width, p.box_height, p.box_depth = 6 + k, 8 - k % 3, 5 + k % 2
+        p.relaxation = 300.0 + 150.0 * k
+        p.enclosure_velocity_damping = 1.0 - 0.1 * k
+        p.box_seed_lo = 11 + k
+        scenes.append(p)
+    return scenes
+
+
+def test_gpu_batch_matches_single_scenes(F, O):
+    # every scene of a batch steps exactly as the oracle steps that scene alone (in the batch's frame)
+    scenes = _batch_scenes(F)
+    flush(f"[batch] {len(scenes)} scenes")
+    b = F.capi.Batch(scenes)
+    b.compute()
+    first = 0
+    for s in range(len(scenes)):
+        cnt, off, wp = b.scene(s)
+        pos, vel, col = F.generate_box(wp)
+        assert cnt == pos.shape[0]
+        out = O.step(wp, O.State(pos, vel, col))
+        h, dt = O.derive(wp).h_spiky, O.derive(wp).dt
+        assert np.array_equal(b.download(s, "cell_ids"), out["keys"])
+        assert np.array_equal(b.download(s, "particle_ids") - first, out["ids"])
+        x = b.download(s, "positions", frame=b.FRAME_BATCH)
+        v = b.download(s, "velocities")
+        ex = np.abs(x[:, :3] - out["x4"][:, :3]).max()
+        ev = np.abs(v[:, :3] - out["v4"][:, :3]).max()
+        flush(f"[batch] scene {s}: n={cnt} max|dx|={ex:.3g} max|dv|={ev:.3g}")
+        assert ex <= 1e-5 * h and ev <= 1e-5 * h / dt
+        rho = out["density"]
+        assert np.abs(b.download(s, "densities") - rho).max() <= 1e-5 * np.abs(rho).max()
+        # the scene frame is the batch frame less the offset
+        xs = b.download(s, "positions")
+        assert np.allclose(xs[:, :3] + off, x[:, :3], rtol=0, atol=1e-5)
+        first += cnt
+    b.close()
+
+
+def test_gpu_batch_graph_and_errors(F, O):
+    scenes = _batch_scenes(F, 3)
+    a, g = F.capi.Batch(scenes), F.capi.Batch(scenes)
+    for _ in range(3 * scenes[0].substeps):
+        a.compute()
+    g.step(3)  # 3 frames: substeps launches of the captured substep each
+    for s in range(3):
+        assert np.array_equal(a.download(s, "positions"), g.download(s, "positions"))
+        assert np.array_equal(a.download(s, "velocities"), g.download(s, "velocities"))
+    a.close()
+    g.close()
+    bad = _batch_scenes(F, 2)
+    bad[1].solver_iterations = 7
+    with pytest.raises(F.PbdError, match="shared parameter"):
+        F.capi.Batch(bad)
+
+
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("case", ["one", "two", "tile_plus_one", "all_in_one_cell", "wide_keys"])
 def test_gpu_integer_stages_edge_cases(F, O, case):
