@@ -13,6 +13,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "pbd-simulation-framework-cuda_amd"))
 
@@ -65,6 +67,77 @@ def measured_traffic(kernel_stage):
     return None
 
 
+def c2_scenes(F, n_scenes, seed=2):
+    """The batched workload: dam-break scenes (the 32k config's solver) with particle
+    counts log-uniform in [2048, 65536], random box aspect, the enclosure sized to the
+    box, and relaxation uniform in [300, 900]."""
+    rng = np.random.default_rng(seed)
+    base = F.load_params(F.config_path("dam_break_32k"))
+    base.delta_t = base.delta_t / base.substeps  # one step = one substep
+    base.substeps = 1
+    base.box_spawn_in_the_beginning = 1
+    s = base.box_spacing
+    scenes = []
+    for k in range(n_scenes):
+        count = float(np.exp(rng.uniform(np.log(2048), np.log(65536))))
+        asp = np.exp(rng.uniform(-0.4, 0.4, 3))
+        side = (count / asp.prod()) ** (1.0 / 3.0)
+        w, h, d = (max(2, int(round(side * a))) for a in asp)
+        p = base.copy()
+        p.box_width, p.box_height, p.box_depth = w, h, d
+        p.box_origin[0] = p.box_origin[1] = p.box_origin[2] = 0.025
+        # fluid block in one corner: 1.5x its length, 1.2x its height, its depth
+        p.bounds_min[0] = p.bounds_min[1] = p.bounds_min[2] = 0.0
+        p.bounds_max[0] = 1.5 * w * s + 0.05
+        p.bounds_max[1] = 1.2 * h * s + 0.05
+        p.bounds_max[2] = d * s + 0.05
+        # the grid of the scene's own context covers its enclosure (a batch uses its own grid)
+        for a in range(3):
+            p.grid_origin[a] = 0.0
+            p.grid_dims[a] = int(np.ceil(p.bounds_max[a] / p.grid_cell_size)) + 1
+        p.relaxation = float(rng.uniform(300.0, 900.0))
+        p.box_seed_lo = 1000 + k
+        scenes.append(p)
+    return scenes
+
+
+def batched_vs_plain(F, n_scenes, steps, warmup):
+    """Sum of N_s x steps / wall time for all scenes in one batch (one graph launch per
+    step) against the same scenes stepped one by one on their own contexts."""
+    scenes = c2_scenes(F, n_scenes)
+    b = F.capi.Batch(scenes)
+    total = sum(b.scene(k)[0] for k in range(n_scenes))
+    b.step(warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    tb = time.perf_counter() - t0
+    b.close()
+    ctxs = [F.Context(p, p.box_width * p.box_height * p.box_depth) for p in scenes]
+    for c in ctxs:
+        c.step(warmup, graph=True)
+        c.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for c in ctxs:
+            c.step(1, graph=True)
+            c.sync()
+    tp = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    return {
+        "scenes": n_scenes,
+        "particles": total,
+        "steps": steps,
+        "batched_particle_steps_per_s": total * steps / tb,
+        "plain_particle_steps_per_s": total * steps / tp,
+        "batched_vs_plain": tp / tb,
+        "workload": f"{n_scenes} dam-break scenes, counts log-uniform in [2048, 65536], random aspect, "
+                    "relaxation U[300, 900]; plain = each scene's own context, one graph launch + sync per step",
+    }
+
+
 def copy_calibration():
     """Three copies of 256 MiB between device buffers (torch's copy kernel)."""
     import torch
@@ -111,6 +184,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--profile-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--scenes", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy (the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters to HBM bytes)")
@@ -171,6 +247,7 @@ def main():
         b = stage_bytes(k, n, p, table, passes)
         v["algorithmic_GBps"] = (b / (v["us_per_launch"] * 1e-6) / 1e9) if v["us_per_launch"] else 0.0
     traffic = measured_traffic(dom)
+    batched = None if args.no_batched else batched_vs_plain(F, args.scenes, args.batched_steps, 3)
 
     line = {
         "metric": METRIC,
@@ -208,6 +285,7 @@ def main():
         "stages": stages,
         "nonfinite_particles": bad,
         "neighbour_lists": counters,
+        "batched": batched,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, args.cpu_steps)
