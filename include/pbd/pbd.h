@@ -268,6 +268,34 @@ enum {
 };
 int pbd_get_counters(pbd_ctx* ctx, uint64_t* values, uint32_t cap, uint32_t* count);
 int pbd_reset_counters(pbd_ctx* ctx);
+/* ---- Reverse mode (no counterpart in the reference) ----
+ * Gradients of a loss L(x_T, v_T) of the state after `frames` frames
+ * (frames x substeps computes of Fluid::compute, src/app/simulation/Fluid.cu:14-97)
+ * with respect to the initial velocities, the rest density and the relaxation
+ * eps (Density.relaxation).  Supported: kernelVariant textbook, collisionType 0
+ * with at most one stabilization iteration, every Active.* stage on, plain
+ * (not batched) contexts; otherwise PBD_ERR_UNSUPPORTED.
+ *
+ * pbd_adj_forward runs the frames (one graph launch per substep) and keeps a
+ * checkpoint of the state (positions, velocities, external forces; 48 B per
+ * particle) at the start of every k_ckpt-th frame.  The context is left in the
+ * final state, in its final particle order. */
+int pbd_adj_forward(pbd_ctx* ctx, uint32_t frames, uint32_t k_ckpt);
+/* For each particle of the final order (count <= number of particles), its
+ * index in the order of the state pbd_adj_forward started from. */
+int pbd_adj_ids(pbd_ctx* ctx, uint32_t* initial_index, size_t count);
+/* dL_dx, dL_dv: float4 per particle (w ignored) in the final order, NULL = 0.
+ * Outputs (NULL = not wanted): dL_dv0 and dL_dx0, float4 per particle in the
+ * initial order (w = 0); dL_drho0 and dL_deps, scalars.  Each segment of
+ * k_ckpt frames is recomputed from its checkpoint while its steps are recorded
+ * (about 13 + 20 x solverIterations bytes per particle and step), then the
+ * reverse kernels run over the same sorted neighbour structure.  The context
+ * is left as pbd_adj_forward left it.  PBD_ERR_UNSUPPORTED if a neighbour list
+ * reached maxNeighboursPerParticle (a capped neighbour relation is not
+ * symmetric). */
+int pbd_adj_backward(pbd_ctx* ctx, const float* dL_dx, const float* dL_dv, float* dL_dv0, float* dL_drho0,
+                     float* dL_deps, float* dL_dx0);
+
 /* Last error message of this context (or of the last failed pbd_create when ctx is NULL). */
 const char* pbd_last_error(pbd_ctx* ctx);
 

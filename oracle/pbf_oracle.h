@@ -115,6 +115,19 @@ uint32_t orc_neighbours(const orc_derived* d, uint32_t n, const uint32_t* keys_s
 
 /* ---- the step ---- */
 int orc_step(const pbd_params* p, orc_io* io);
+/* ---- FP64 multi-step forward and its reverse mode (pbf_adjoint.c) ----
+ * Textbook variant, collisionType 0, at most one stabilization pass, all
+ * stages on (else PBD_ERR_UNSUPPORTED); a neighbour list reaching the cap is
+ * PBD_ERR_UNSUPPORTED as well.  State: n x 4 doubles (w ignored).  rho0 <= 0
+ * and eps < 0 select the values derived from the parameters. */
+int orc64_forward(const pbd_params* p, double rho0, double eps, uint32_t n, uint32_t nsub, double* x4,
+                  double* v4, double* f4, uint32_t* orig);
+/* dL/dx0, dL/dv0 (order of the initial state), dL/drho0, dL/deps of a loss with
+ * adjoints gx4, gv4 of the final positions and velocities (final order). */
+int orc64_gradient(const pbd_params* p, double rho0, double eps, uint32_t n, uint32_t nsub, const double* x4,
+                   const double* v4, const double* f4, const double* gx4, const double* gv4, double* gx0,
+                   double* gv0, double* grho0, double* geps);
+
 /* Number of OpenMP threads the oracle uses (1 when built without OpenMP). */
 int orc_threads(void);
 void orc_set_threads(int n);

@@ -1,0 +1,422 @@
+// adjoint.hip - gfx950 kernels of the reverse mode of the PBF step.
+//
+// Forward step (textbook variant, collisionType 0, at most one stabilization
+// pass), as the kernels in kernels.hip / tiles.hip compute it:
+//   predict     v' = v + dt (g y + f), p = x + dt v', box clamp      Forces.h:19-101
+//   sort        cell ids, stable sort, reorder                        SortReorder.h
+//   iterate     lambda_i = -C_i / D_i,  C_i = rho_i/rho0 - 1,
+//               D_i = (|sum_j G_ij|^2 + sum_j |G_ij|^2)/rho0^2 + eps  Density.h:48-93
+//               dp_i = 1/rho0 sum_j (lambda_i + lambda_j + s_corr(W_ij)) G_ij   Density.h:104-145
+//               push_i = sum_{contacts} k_c (d - r)/r r_ij            Collision.h:171-242
+//               p <- clamp(p + push + dp), x <- x + push              Density.h:156-168
+//   update      u = (p - x)/dt, x = p                                 Update.h:11-32
+//   omega       w_i = 1/rho0 sum_j G_ij x (u_j - u_i)                 Density.h:232-269
+//   vorticity   eta_i = sum_j (|w_j| - |w_i|) G_ij, f_i = e (eta_i/|eta_i|) x w_i   Density.h:179-221
+//   XSPH        v_i = u_i + c/rho0 sum_j (u_j - u_i) W_ij             Density.h:280-310
+// W = poly6 (true units), G_ij = grad_i W_spiky(x_i - x_j), H_ij = dG_ij/dx_i
+// = gc psi I - gc (h - r)(h + r)/r^3 r r^T with psi = (h - r)^2/r.
+//
+// Every neighbour relation is symmetric (the lists stay below the cap; the
+// kernels flag it otherwise), so the adjoint of a pair sum is a gather over
+// the same neighbours combining the i-side and j-side adjoints, e.g.
+//   d/dx_i sum_k a_k . sum_j G_kj = sum_j H_ij (a_i - a_j).
+// Passes per step (last stage first), each one kernel over the particles:
+//   A  XSPH (self-adjoint gather), eta and the vorticity-force adjoint
+//   B  |omega| adjoint through eta; position terms of eta
+//   C  omega adjoint, the update, the clamp mask of the last iteration
+//   R1 per iteration: lambda adjoint from dp, position terms of dp and the
+//      push; adjoints of rho_i, a_i, b_i
+//   R2 per iteration: position terms of rho, a and b; clamp mask of the
+//      previous iteration
+//   predict, scattered back through the sort permutation.
+// Nonsmooth points (clamps, contact onset, kernel supports, eta = 0) take the
+// derivative of the branch the forward took.
+#include <hip/hip_runtime.h>
+
+#include "adjoint.hpp"
+#include "device.hpp"
+
+namespace pbd {
+
+namespace {
+
+constexpr uint32_t kAdjThreads = 256;
+
+// Pair terms of r = x_i - x_j: W, grad W = wgs r, G = gs r, H v = gs v - hb (r.v) r.
+struct AdjPair {
+  float W, wgs, gs, hb, len, ir;
+};
+
+__device__ __forceinline__ AdjPair adj_pair(const DevParams& P, float r2) {
+  AdjPair q;
+  const float t = fmaxf(P.hp2 - r2, 0.0f);
+  q.W = P.wc * (t * t * t);
+  q.wgs = -6.0f * P.wc * (t * t);
+  const float r2c = fmaxf(r2, 1.0e-30f);
+  const float ir = rsq(r2c);
+  const float len = r2c * ir;
+  const float hr = fmaxf(P.hs - len, 0.0f);
+  const bool pos = r2 > 0.0f;
+  q.gs = pos ? P.gc * (hr * hr * ir) : 0.0f;
+  q.hb = pos ? P.gc * hr * (P.hs + len) * (ir * ir * ir) : 0.0f;
+  q.len = len;
+  q.ir = ir;
+  return q;
+}
+
+__device__ __forceinline__ float3 f3(float4 a) { return make_float3(a.x, a.y, a.z); }
+__device__ __forceinline__ float3 operator+(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 operator-(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 operator*(float s, float3 a) { return make_float3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ void operator+=(float3& a, float3 b) {
+  a.x += b.x;
+  a.y += b.y;
+  a.z += b.z;
+}
+__device__ __forceinline__ float dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross(float3 a, float3 b) {
+  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// H v
+__device__ __forceinline__ float3 hmul(const AdjPair& q, float3 r, float3 v) {
+  const float rv = q.hb * dot(r, v);
+  return make_float3(q.gs * v.x - rv * r.x, q.gs * v.y - rv * r.y, q.gs * v.z - rv * r.z);
+}
+
+// s_corr(W) = -k (W / W(dq))^n and its derivative in W
+__device__ __forceinline__ float scorr_of(const DevParams& P, float W) { return -P.k_scorr * ipow(W * P.inv_w_dq, P.n_scorr); }
+__device__ __forceinline__ float dscorr_of(const DevParams& P, float W) {
+  if (P.n_scorr == 0) return 0.0f;
+  return -P.k_scorr * (float)P.n_scorr * ipow(W * P.inv_w_dq, P.n_scorr - 1) * P.inv_w_dq;
+}
+
+// Axes on which the projection onto the enclosure (delta_finish) clamped the
+// position q of the particle in cell `key`: the clamp writes the bound itself.
+__device__ __forceinline__ uint32_t clamp_mask(const DevParams& P, uint32_t key, float4 q) {
+  const float cx = cell_corner(P.origin[0], P.cell, compact10(key >> 2));
+  const float cy = cell_corner(P.origin[1], P.cell, compact10(key >> 1));
+  const float cz = cell_corner(P.origin[2], P.cell, compact10(key));
+  const ProjBox b = scene_box(P, key);
+  uint32_t m = 0;
+  if (q.x == b.lo[0] - cx || q.x == b.hi[0] - cx) m |= 1u;
+  if (q.y == b.lo[1] - cy || q.y == b.hi[1] - cy) m |= 2u;
+  if (q.z == b.lo[2] - cz || q.z == b.hi[2] - cz) m |= 4u;
+  return m;
+}
+
+__device__ __forceinline__ float4 masked(float3 v, uint32_t m) {
+  return make_float4((m & 1u) ? 0.0f : v.x, (m & 2u) ? 0.0f : v.y, (m & 4u) ? 0.0f : v.z, 0.0f);
+}
+
+__device__ __forceinline__ bool shells_on(const DevParams& P) { return (P.gates & G_FIND_NEIGHBOURS) != 0; }
+
+// ---------------------------------------------------------------- A
+__global__ void __launch_bounds__(kAdjThreads)
+k_adj_a(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
+        const uint32_t* __restrict__ ce, const float4* __restrict__ U, const float4* __restrict__ OM,
+        const float4* __restrict__ GV, const float4* __restrict__ GF, float4* __restrict__ Ub,
+        float4* __restrict__ Xp, float4* __restrict__ Eb, float4* __restrict__ Ob, double2* __restrict__ acc,
+        uint32_t* __restrict__ flag) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const float4 qi = Q[s], oi4 = OM[s];
+  const float3 ui = f3(U[s]), gvi = f3(GV[s]), oi = f3(oi4);
+  const CellFrame f = make_frame(P, keys[s]);
+  const float cv = P.c_visc * P.vol;
+  float3 ub = gvi, xb = make_float3(0.f, 0.f, 0.f), eta = xb, xs = xb;
+  uint32_t nc = 0;
+  const uint32_t cnt = visit_neighbours(
+      P, s, f, qi.x, qi.y, qi.z, true, shells_on(P), cs, ce, nc, [&](uint32_t j, float bx, float by, float bz, bool) {
+        const float4 qj = Q[j];
+        const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
+        const AdjPair q = adj_pair(P, dot(r, r));
+        const float3 du = f3(U[j]) - ui, dg = f3(GV[j]) - gvi;
+        ub += (cv * q.W) * dg;
+        xs += q.W * du;
+        xb += (-cv * dot(du, dg) * q.wgs) * r;
+        eta += ((OM[j].w - oi4.w) * q.gs) * r;
+      });
+  if (cnt >= P.cap) flag[0] = 1u;
+  acc[s].x += (double)(-cv * dot(gvi, xs) * P.inv_rho0);
+  const float el = sqrtf(dot(eta, eta));
+  const float inv = el > 0.0f ? 1.0f / el : 0.0f;
+  const float3 N = inv * eta, gf = f3(GF[s]);
+  const float3 ob = P.e_vort * cross(gf, N);
+  const float3 nb = P.e_vort * cross(oi, gf);
+  const float3 eb = inv * (nb - dot(N, nb) * N);
+  Ub[s] = make_float4(ub.x, ub.y, ub.z, 0.0f);
+  Xp[s] = make_float4(xb.x, xb.y, xb.z, 0.0f);
+  Eb[s] = make_float4(eb.x, eb.y, eb.z, 0.0f);
+  Ob[s] = make_float4(ob.x, ob.y, ob.z, 0.0f);
+}
+
+// ---------------------------------------------------------------- B
+__global__ void __launch_bounds__(kAdjThreads)
+k_adj_b(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
+        const uint32_t* __restrict__ ce, const float4* __restrict__ OM, const float4* __restrict__ Eb,
+        float4* __restrict__ Ob, float4* __restrict__ Xp) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const float4 qi = Q[s], oi4 = OM[s];
+  const float3 ebi = f3(Eb[s]);
+  const CellFrame f = make_frame(P, keys[s]);
+  float mb = 0.0f;
+  float3 xb = make_float3(0.f, 0.f, 0.f);
+  uint32_t nc = 0;
+  visit_neighbours(P, s, f, qi.x, qi.y, qi.z, true, shells_on(P), cs, ce, nc,
+                   [&](uint32_t j, float bx, float by, float bz, bool) {
+                     const float4 qj = Q[j];
+                     const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
+                     const AdjPair q = adj_pair(P, dot(r, r));
+                     const float3 es = ebi + f3(Eb[j]);
+                     mb -= q.gs * dot(es, r);
+                     xb += (OM[j].w - oi4.w) * hmul(q, r, es);
+                   });
+  const float4 x = Xp[s];
+  Xp[s] = make_float4(x.x + xb.x, x.y + xb.y, x.z + xb.z, 0.0f);
+  float3 ob = f3(Ob[s]);
+  if (oi4.w > 0.0f) ob += (mb / oi4.w) * f3(oi4);
+  Ob[s] = make_float4(ob.x, ob.y, ob.z, 0.0f);
+}
+
+// ---------------------------------------------------------------- C
+__global__ void __launch_bounds__(kAdjThreads)
+k_adj_c(DevParams P, uint32_t stab, const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
+        const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce, const float4* __restrict__ U,
+        const float4* __restrict__ OM, const float4* __restrict__ Ob, const float4* __restrict__ Ub,
+        const float4* __restrict__ Xp, const float4* __restrict__ GX, float4* __restrict__ Xb,
+        float4* __restrict__ DPb, float4* __restrict__ PUb, double2* __restrict__ acc) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const float4 qi = Q[s];
+  const uint32_t key = keys[s];
+  const float3 ui = f3(U[s]), obi = f3(Ob[s]);
+  const CellFrame f = make_frame(P, key);
+  float3 ub = make_float3(0.f, 0.f, 0.f), xb = ub;
+  uint32_t nc = 0;
+  visit_neighbours(P, s, f, qi.x, qi.y, qi.z, true, shells_on(P), cs, ce, nc,
+                   [&](uint32_t j, float bx, float by, float bz, bool) {
+                     const float4 qj = Q[j];
+                     const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
+                     const AdjPair q = adj_pair(P, dot(r, r));
+                     const float3 os = obi + f3(Ob[j]);
+                     ub += (-P.inv_rho0 * q.gs) * cross(os, r);
+                     xb += P.inv_rho0 * hmul(q, r, cross(f3(U[j]) - ui, os));
+                   });
+  acc[s].x += (double)(-dot(obi, f3(OM[s])) * P.inv_rho0);
+  const float3 u = f3(Ub[s]) + ub;
+  const float3 pb = f3(GX[s]) + f3(Xp[s]) + xb + P.inv_dt * u;
+  const float3 xv = -P.inv_dt * u;
+  Xb[s] = make_float4(xv.x, xv.y, xv.z, 0.0f);
+  const float4 d = masked(pb, clamp_mask(P, key, qi));
+  DPb[s] = d;
+  PUb[s] = stab ? make_float4(d.x + xv.x, d.y + xv.y, d.z + xv.z, 0.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---------------------------------------------------------------- R1
+__global__ void __launch_bounds__(kAdjThreads)
+k_adj_r1(DevParams P, uint32_t stab, const float4* __restrict__ Q, const float* __restrict__ LAM,
+         const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+         const float4* __restrict__ DPb, const float4* __restrict__ PUb, float4* __restrict__ Pn,
+         float4* __restrict__ AB, float* __restrict__ Rb, double2* __restrict__ acc, uint32_t* __restrict__ flag) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const float4 qi = Q[s];
+  const uint32_t key = keys[s];
+  const float li = LAM[s];
+  const float3 dpi = f3(DPb[s]), pui = f3(PUb[s]);
+  const CellFrame f = make_frame(P, key);
+  float A = 0.0f, B = 0.0f, lb = 0.0f;
+  float3 a = make_float3(0.f, 0.f, 0.f), dsum = a, pn = dpi;
+  uint32_t nc = 0;
+  const uint32_t cnt = visit_neighbours(
+      P, s, f, qi.x, qi.y, qi.z, true, shells_on(P), cs, ce, nc,
+      [&](uint32_t j, float bx, float by, float bz, bool contact) {
+        const float4 qj = Q[j];
+        const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
+        const float r2 = dot(r, r);
+        const AdjPair q = adj_pair(P, r2);
+        const float3 G = q.gs * r;
+        A += q.W;
+        a += G;
+        B += q.gs * q.gs * r2;
+        const float3 dd = dpi - f3(DPb[j]);
+        const float gd = dot(G, dd);
+        lb += gd;
+        const float S = li + LAM[j] + scorr_of(P, q.W);
+        dsum += S * G;
+        pn += (S * P.inv_rho0) * hmul(q, r, dd);
+        pn += (dscorr_of(P, q.W) * gd * P.inv_rho0 * q.wgs) * r;
+        if (stab && contact && q.len < P.diameter && q.len > 0.00001f) {
+          const float3 pd = pui - f3(PUb[j]);
+          const float phi = (P.diameter - q.len) * q.ir;
+          const float rr = dot(r, pd) * P.diameter * (q.ir * q.ir * q.ir);
+          const float kc = -P.coll_scale;
+          pn += kc * (phi * pd - rr * r);
+        }
+      });
+  if (cnt >= P.cap) flag[0] = 1u;
+  lb *= P.inv_rho0;
+  const float rho = A + P.w0;
+  const float s2 = (dot(a, a) + B) * P.inv_rho0_sq;
+  const float D = s2 + scene_eps(P, key);
+  const float Cc = rho * P.inv_rho0 - 1.0f;
+  const float cb = -lb / D;
+  const float db = lb * Cc / (D * D);
+  double2 g = acc[s];
+  g.x += (double)(-dot(dpi, dsum) * P.inv_rho0_sq) + (double)(cb * (-rho * P.inv_rho0_sq)) +
+         (double)(db * (-2.0f * s2 * P.inv_rho0));
+  g.y += (double)db;
+  acc[s] = g;
+  Rb[s] = cb * P.inv_rho0;
+  const float da = 2.0f * db * P.inv_rho0_sq;
+  AB[s] = make_float4(da * a.x, da * a.y, da * a.z, db * P.inv_rho0_sq);
+  Pn[s] = make_float4(pn.x, pn.y, pn.z, 0.0f);
+}
+
+// ---------------------------------------------------------------- R2
+// last: the first iteration (it = 0): Pb <- the adjoint of its input
+// positions; otherwise DPb / PUb <- the masked adjoint for iteration it - 1.
+__global__ void __launch_bounds__(kAdjThreads)
+k_adj_r2(DevParams P, uint32_t stab, uint32_t last, const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
+         const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce, const float4* __restrict__ AB,
+         const float* __restrict__ Rb, const float4* __restrict__ Pn, const float4* __restrict__ Xb,
+         float4* __restrict__ DPb, float4* __restrict__ PUb, float4* __restrict__ Pb) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  const float4 qi = Q[s];
+  const uint32_t key = keys[s];
+  const float4 abi = AB[s];
+  const float rbi = Rb[s];
+  const CellFrame f = make_frame(P, key);
+  float3 pn = f3(Pn[s]);
+  uint32_t nc = 0;
+  visit_neighbours(P, s, f, qi.x, qi.y, qi.z, true, shells_on(P), cs, ce, nc,
+                   [&](uint32_t j, float bx, float by, float bz, bool) {
+                     const float4 qj = Q[j];
+                     const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
+                     const AdjPair q = adj_pair(P, dot(r, r));
+                     const float4 abj = AB[j];
+                     pn += ((rbi + Rb[j]) * q.wgs) * r;
+                     pn += hmul(q, r, f3(abi) - f3(abj) + (2.0f * (abi.w + abj.w) * q.gs) * r);
+                   });
+  if (last) {
+    Pb[s] = make_float4(pn.x, pn.y, pn.z, 0.0f);
+    return;
+  }
+  const float4 d = masked(pn, clamp_mask(P, key, qi));
+  DPb[s] = d;
+  const float4 xv = Xb[s];
+  PUb[s] = stab ? make_float4(d.x + xv.x, d.y + xv.y, d.z + xv.z, 0.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---------------------------------------------------------------- predict
+// Per axis: the forward's clamp flags, recomputed with the predict stage's
+// FP32 operations (device.hpp predict).
+__global__ void __launch_bounds__(kAdjThreads)
+k_adj_predict(DevParams P, const uint32_t* __restrict__ perm, const float4* __restrict__ Xin,
+              const float4* __restrict__ Vin, const float4* __restrict__ Fin, const float4* __restrict__ Pb,
+              const float4* __restrict__ Xb, float4* __restrict__ GX, float4* __restrict__ GV,
+              float4* __restrict__ GF) {
+#pragma clang fp contract(off)
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  uint32_t j = perm[s];
+  if (j >= P.n) j = P.n - 1u;
+  const float4 x = Xin[j], f = Fin[j];
+  float4 v = Vin[j];
+  const Enclosure E = enclosure_of(P, x);
+  const float dt = P.dt;
+  const float gdt = (1.0f * P.gravity) * dt;
+  v.y = v.y + gdt;
+  v.x = v.x + f.x * dt;
+  v.y = v.y + f.y * dt;
+  v.z = v.z + f.z * dt;
+  const float px = x.x + v.x * dt, py = x.y + v.y * dt, pz = x.z + v.z * dt;
+  const bool cx = px < E.bmin[0] || px > E.bmax[0];
+  const bool cy = py < E.bmin[1] || py > E.bmax[1];
+  const bool cz = pz < E.bmin[2] || pz > E.bmax[2];
+  const float pd = (cx || cy || cz) ? E.pdamp : 0.0f;
+  const float4 pb = Pb[s], xb = Xb[s];
+  const float tx = cx ? 0.0f : pb.x + pd * xb.x;
+  const float ty = cy ? 0.0f : pb.y + pd * xb.y;
+  const float tz = cz ? 0.0f : pb.z + pd * xb.z;
+  const float k = 1.0f - pd;
+  GX[j] = make_float4(k * xb.x + tx, k * xb.y + ty, k * xb.z + tz, 0.0f);
+  GV[j] = make_float4(dt * tx, dt * ty, dt * tz, 0.0f);
+  GF[j] = make_float4(dt * dt * tx, dt * dt * ty, dt * dt * tz, 0.0f);
+}
+
+__global__ void k_gather_ids(uint32_t n, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ in,
+                             uint32_t* __restrict__ out) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  uint32_t j = perm[s];
+  if (j >= n) j = n - 1u;
+  out[s] = in[j];
+}
+
+constexpr uint32_t kReduceThreads = 1024;
+__global__ void __launch_bounds__(kReduceThreads)
+k_adj_reduce(uint32_t n, const double2* __restrict__ acc, double* __restrict__ out) {
+  __shared__ double sx[kReduceThreads], sy[kReduceThreads];
+  double x = 0.0, y = 0.0;
+  for (uint32_t i = threadIdx.x; i < n; i += kReduceThreads) {
+    const double2 a = acc[i];
+    x += a.x;
+    y += a.y;
+  }
+  sx[threadIdx.x] = x;
+  sy[threadIdx.x] = y;
+  __syncthreads();
+  for (uint32_t w = kReduceThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sx[threadIdx.x] += sx[threadIdx.x + w];
+      sy[threadIdx.x] += sy[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sx[0];
+    out[1] = sy[0];
+  }
+}
+
+}  // namespace
+
+hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P, uint32_t iters, uint32_t stab,
+                               const AdjTape& T, const AdjWork& W) {
+  const uint32_t n = P.n;
+  if (n == 0) return hipSuccess;
+  if (iters == 0) return hipErrorInvalidValue;  // the host rejects steps without solver iterations
+  const dim3 grid((n + kAdjThreads - 1) / kAdjThreads), block(kAdjThreads);
+  const float4* Qf = T.Q + (size_t)iters * n;
+  k_adj_a<<<grid, block, 0, st>>>(P, Qf, T.keys, T.cs, T.ce, T.U, T.OM, W.GV, W.GF, W.Ub, W.Xp, W.Eb, W.Ob, W.acc,
+                                  W.flag);
+  k_adj_b<<<grid, block, 0, st>>>(P, Qf, T.keys, T.cs, T.ce, T.OM, W.Eb, W.Ob, W.Xp);
+  k_adj_c<<<grid, block, 0, st>>>(P, stab, Qf, T.keys, T.cs, T.ce, T.U, T.OM, W.Ob, W.Ub, W.Xp, W.GX, W.Xb, W.DPb,
+                                  W.PUb, W.acc);
+  for (uint32_t it = iters; it-- > 0;) {
+    const float4* Q = T.Q + (size_t)it * n;
+    k_adj_r1<<<grid, block, 0, st>>>(P, stab, Q, T.LAM + (size_t)it * n, T.keys, T.cs, T.ce, W.DPb, W.PUb, W.Pn,
+                                     W.AB, W.Rb, W.acc, W.flag);
+    k_adj_r2<<<grid, block, 0, st>>>(P, stab, it == 0 ? 1u : 0u, Q, T.keys, T.cs, T.ce, W.AB, W.Rb, W.Pn, W.Xb,
+                                     W.DPb, W.PUb, W.Pb);
+  }
+  k_adj_predict<<<grid, block, 0, st>>>(P, T.perm, T.Xin, T.Vin, T.Fin, W.Pb, W.Xb, W.GX, W.GV, W.GF);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_ids(hipStream_t st, uint32_t n, const uint32_t* perm, const uint32_t* ids_in,
+                             uint32_t* ids_out) {
+  if (n == 0) return hipSuccess;
+  k_gather_ids<<<(n + 255) / 256, 256, 0, st>>>(n, perm, ids_in, ids_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_reduce(hipStream_t st, uint32_t n, const double2* acc, double* out) {
+  k_adj_reduce<<<1, kReduceThreads, 0, st>>>(n, acc, out);
+  return hipGetLastError();
+}
+
+}  // namespace pbd

@@ -1,0 +1,65 @@
+// adjoint.hpp - reverse mode of the PBF step (textbook variant, collisionType 0).
+//
+// The forward step is Fluid::compute (src/app/simulation/Fluid.cu:14-97 as
+// restated in fluid.cpp).  One step's record (AdjTape) holds what the reverse
+// passes read: the positions at the start of every solver iteration, the
+// lambdas, the velocities after the update, the vorticity, the sort
+// permutation, the cell table and the state the step started from.  The
+// reverse kernels walk the same sorted neighbour structure (cell table of the
+// step) as the forward.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "params.hpp"
+
+namespace pbd {
+
+// Record of one step (sorted order of the step unless noted).
+struct AdjTape {
+  float4* Q;       // (iters + 1) x n: positions (cell-local) at each iteration's start; [iters] = final
+  float* LAM;      // iters x n
+  float4* U;       // n: velocities after the update, before XSPH
+  float4* OM;      // n: vorticity (w = |omega|)
+  uint32_t* keys;  // n: sorted cell ids
+  uint32_t* perm;  // n: slot -> index before the sort
+  uint32_t* cs;    // table: cell starts / ends
+  uint32_t* ce;
+  float4* Xin;     // n: the step's input state, order before the sort
+  float4* Vin;
+  float4* Fin;
+};
+
+// Adjoint work buffers (n each, sorted order).
+struct AdjWork {
+  float4* GX;   // in: d L / d positions out, d L / d velocities out, d L / d external forces out;
+  float4* GV;   // out (after the predict pass): the same for the step's inputs, order before the sort
+  float4* GF;
+  float4* Ub;   // velocity-after-update adjoint
+  float4* Xp;   // position adjoint of the output stages
+  float4* Eb;   // eta adjoint
+  float4* Ob;   // omega adjoint
+  float4* Pb;   // predicted-position adjoint
+  float4* Xb;   // position (x) adjoint, constant through the iterations
+  float4* DPb;  // delta-p adjoint (clamp-masked)
+  float4* PUb;  // push adjoint
+  float4* AB;   // adjoint of a_i = sum_j grad W_ij (xyz) and of b_i = sum_j |grad W_ij|^2 (w)
+  float4* Pn;   // partial predicted-position adjoint of an iteration
+  float* Rb;    // density adjoint
+  double2* acc; // per slot: accumulated d L / d rho0 (x) and d L / d eps (y)
+  uint32_t* flag;  // [0]: a neighbour list reached the cap
+};
+
+// Reverse of one step: W.GX/GV/GF hold the adjoints of the step's outputs on
+// entry and those of its inputs on return.  stab: stabilization passes (0 or 1).
+hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P, uint32_t iters, uint32_t stab,
+                               const AdjTape& T, const AdjWork& W);
+// ids_out[s] = ids_in[perm[s]] (identity of the particles through a sort)
+hipError_t launch_gather_ids(hipStream_t st, uint32_t n, const uint32_t* perm, const uint32_t* ids_in,
+                             uint32_t* ids_out);
+// out = sum over slots of W.acc (one block; out in device memory)
+hipError_t launch_adjoint_reduce(hipStream_t st, uint32_t n, const double2* acc, double* out);
+
+}  // namespace pbd

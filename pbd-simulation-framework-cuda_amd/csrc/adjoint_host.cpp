@@ -1,0 +1,207 @@
+// adjoint_host.cpp - host side of the reverse mode: checkpoints, segment
+// recomputation with a step record, and the reverse passes (adjoint.hip).
+//
+// adj_forward runs `frames` frames (frames x substeps computes, one graph
+// launch each) and keeps the state (positions, velocities, external forces) at
+// the start of every k-th frame.  adj_backward walks the segments last first:
+// it restores the segment's checkpoint, recomputes its steps eagerly while
+// every step writes its record (AdjTape), and runs the reverse kernels of the
+// steps last first.  The recomputation replays the same kernels on the same
+// inputs, so it reproduces the forward bit for bit.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <sstream>
+#include <vector>
+
+#include "fluid.hpp"
+
+namespace pbd {
+
+struct Fluid::AdjState {
+  uint32_t n = 0, frames = 0, k = 0, sub = 0, iters = 0, stab = 0, nseg = 0, nrec = 0, table = 0;
+  std::vector<void*> mem;
+  std::vector<float4*> ckpt;    // per checkpoint: X | V | F (3n float4)
+  float4* final_state = nullptr;  // X | P | V | C | F after the forward
+  std::vector<AdjTape> tape;    // records of one segment
+  AdjWork W{};
+  uint32_t* ids[2] = {nullptr, nullptr};
+  int ids_cur = 0;
+  double* red = nullptr;  // 2 doubles
+  bool valid = false;
+  pbd_params params{};
+
+  ~AdjState() {
+    for (void* p : mem) (void)hipFree(p);
+  }
+  template <class T>
+  bool alloc(T** p, size_t count) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return false;
+    mem.push_back(q);
+    *p = static_cast<T*>(q);
+    return true;
+  }
+};
+
+void Fluid::adj_release() {
+  delete adj_;
+  adj_ = nullptr;
+}
+
+// The reverse kernels cover the textbook step with particle collisions of
+// type 0 (at most one stabilization pass) and every stage active.
+int Fluid::adj_check() {
+  const pbd_params& p = params_;
+  if (p.kernel_variant != PBD_KERNEL_TEXTBOOK)
+    return fail(PBD_ERR_UNSUPPORTED, "adjoint: only Density.kernelVariant = textbook is differentiated");
+  if (p.collision_type != 0 || p.stabilization_iterations > 1)
+    return fail(PBD_ERR_UNSUPPORTED, "adjoint: collisionType 0 with at most one stabilization iteration");
+  if (p.solver_iterations == 0) return fail(PBD_ERR_UNSUPPORTED, "adjoint: at least one solver iteration");
+  if (!p.active_apply_forces || !p.active_find_contacts || !p.active_compute_lambda ||
+      !p.active_compute_delta_positions || !p.active_apply_delta_positions || !p.active_update_positions ||
+      !p.active_compute_omegas || !p.active_compute_vorticity || !p.active_compute_viscosity)
+    return fail(PBD_ERR_UNSUPPORTED, "adjoint: every Active.* stage but findNeighbours must be on");
+  if (scene_table_) return fail(PBD_ERR_UNSUPPORTED, "adjoint: not available for batched scenes");
+  if (n_ == 0) return fail(PBD_ERR_INVALID, "adjoint: no particles");
+  return PBD_OK;
+}
+
+int Fluid::adj_forward(uint32_t frames, uint32_t k_ckpt) {
+  if (status_ != PBD_OK) return status_;
+  if (int rc = adj_check()) return rc;
+  if (frames == 0 || k_ckpt == 0) return fail(PBD_ERR_INVALID, "adjoint: frames and the checkpoint interval must be > 0");
+  if (int rc = sync()) return rc;
+  if (!dev_valid_) {
+    const char* why = nullptr;
+    if (derive_device_params(params_, n_, dev_, &why) != PBD_OK) return fail(PBD_ERR_INVALID, why);
+    apply_scenes();
+    dev_valid_ = true;
+  }
+  adj_release();
+  adj_ = new AdjState();
+  AdjState& A = *adj_;
+  A.n = n_;
+  A.frames = frames;
+  A.k = std::min(k_ckpt, frames);
+  A.sub = params_.substeps ? params_.substeps : 1;
+  A.iters = params_.solver_iterations;
+  A.stab = params_.stabilization_iterations;
+  A.nseg = (frames + A.k - 1) / A.k;
+  A.nrec = A.k * A.sub;
+  A.table = B_.table_size;
+  A.params = params_;
+  const size_t n = n_;
+  bool ok = true;
+  A.ckpt.assign(A.nseg, nullptr);
+  for (uint32_t g = 0; g < A.nseg && ok; ++g) ok = A.alloc(&A.ckpt[g], 3 * n);
+  ok = ok && A.alloc(&A.final_state, 5 * n);
+  A.tape.assign(A.nrec, AdjTape{});
+  for (uint32_t t = 0; t < A.nrec && ok; ++t) {
+    AdjTape& T = A.tape[t];
+    ok = A.alloc(&T.Q, (size_t)(A.iters + 1) * n) && A.alloc(&T.LAM, (size_t)A.iters * n) && A.alloc(&T.U, n) &&
+         A.alloc(&T.OM, n) && A.alloc(&T.keys, n) && A.alloc(&T.perm, n) && A.alloc(&T.cs, A.table) &&
+         A.alloc(&T.ce, A.table) && A.alloc(&T.Xin, n) && A.alloc(&T.Vin, n) && A.alloc(&T.Fin, n);
+  }
+  AdjWork& W = A.W;
+  ok = ok && A.alloc(&W.GX, n) && A.alloc(&W.GV, n) && A.alloc(&W.GF, n) && A.alloc(&W.Ub, n) && A.alloc(&W.Xp, n) &&
+       A.alloc(&W.Eb, n) && A.alloc(&W.Ob, n) && A.alloc(&W.Pb, n) && A.alloc(&W.Xb, n) && A.alloc(&W.DPb, n) &&
+       A.alloc(&W.PUb, n) && A.alloc(&W.AB, n) && A.alloc(&W.Pn, n) && A.alloc(&W.Rb, n) && A.alloc(&W.acc, n) &&
+       A.alloc(&W.flag, 1) && A.alloc(&A.ids[0], n) && A.alloc(&A.ids[1], n) && A.alloc(&A.red, 2);
+  if (!ok) {
+    adj_release();
+    return fail(PBD_ERR_NOMEM, "adjoint: device allocation of checkpoints and step records failed");
+  }
+  std::vector<uint32_t> iota(n);
+  std::iota(iota.begin(), iota.end(), 0u);
+  if (hip_check(hipMemcpy(A.ids[0], iota.data(), n * 4, hipMemcpyHostToDevice), "adjoint ids")) return status_;
+  A.ids_cur = 0;
+  auto save = [&](float4* dst) {
+    return tape_copy(dst, B_.X, n * 16) || tape_copy(dst + n, B_.V, n * 16) || tape_copy(dst + 2 * n, B_.F, n * 16);
+  };
+  if (save(A.ckpt[0])) return status_;
+  if (int rc = ensure_graph()) return rc;
+  for (uint32_t f = 0; f < frames; ++f) {
+    for (uint32_t s = 0; s < A.sub; ++s) {
+      if (hip_check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch")) return status_;
+      if (hip_check(launch_gather_ids(stream_, n_, ids_sorted_, A.ids[A.ids_cur], A.ids[1 - A.ids_cur]), "ids"))
+        return status_;
+      A.ids_cur = 1 - A.ids_cur;
+    }
+    if ((f + 1) % A.k == 0 && f + 1 < frames && save(A.ckpt[(f + 1) / A.k])) return status_;
+  }
+  float4* fs = A.final_state;
+  if (tape_copy(fs, B_.X, n * 16) || tape_copy(fs + n, B_.P, n * 16) || tape_copy(fs + 2 * n, B_.V, n * 16) ||
+      tape_copy(fs + 3 * n, B_.C, n * 16) || tape_copy(fs + 4 * n, B_.F, n * 16))
+    return status_;
+  A.valid = true;
+  return PBD_OK;
+}
+
+int Fluid::adj_ids(uint32_t* dst, size_t count) {
+  if (!adj_ || !adj_->valid) return fail(PBD_ERR_INVALID, "adjoint: run pbd_adj_forward first");
+  if (count > adj_->n || (count && !dst)) return fail(PBD_ERR_INVALID, "adjoint: ids count exceeds particle count");
+  if (int rc = sync()) return rc;
+  if (count == 0) return PBD_OK;
+  return hip_check(hipMemcpy(dst, adj_->ids[adj_->ids_cur], count * 4, hipMemcpyDeviceToHost), "adjoint ids");
+}
+
+int Fluid::adj_backward(const float* gx, const float* gv, float* gv0, float* grho0, float* geps, float* gx0) {
+  if (status_ != PBD_OK) return status_;
+  if (!adj_ || !adj_->valid) return fail(PBD_ERR_INVALID, "adjoint: run pbd_adj_forward first");
+  AdjState& A = *adj_;
+  if (A.n != n_ || std::memcmp(&A.params, &params_, sizeof(pbd_params)) != 0)
+    return fail(PBD_ERR_INVALID, "adjoint: particles or parameters changed since pbd_adj_forward");
+  const size_t n = n_;
+  const AdjWork& W = A.W;
+  hipError_t e = hipSuccess;
+  if (gx) e = hipMemcpyAsync(W.GX, gx, n * 16, hipMemcpyHostToDevice, stream_);
+  else e = hipMemsetAsync(W.GX, 0, n * 16, stream_);
+  if (e == hipSuccess) {
+    if (gv) e = hipMemcpyAsync(W.GV, gv, n * 16, hipMemcpyHostToDevice, stream_);
+    else e = hipMemsetAsync(W.GV, 0, n * 16, stream_);
+  }
+  if (e == hipSuccess) e = hipMemsetAsync(W.GF, 0, n * 16, stream_);
+  if (e == hipSuccess) e = hipMemsetAsync(W.acc, 0, n * sizeof(double2), stream_);
+  if (e == hipSuccess) e = hipMemsetAsync(W.flag, 0, 4, stream_);
+  if (hip_check(e, "adjoint: upload of the loss adjoints")) return status_;
+  for (uint32_t g = A.nseg; g-- > 0;) {
+    const float4* c = A.ckpt[g];
+    if (tape_copy(B_.X, c, n * 16) || tape_copy(B_.V, c + n, n * 16) || tape_copy(B_.F, c + 2 * n, n * 16))
+      return status_;
+    const uint32_t nstep = (std::min(A.frames, (g + 1) * A.k) - g * A.k) * A.sub;
+    for (uint32_t t = 0; t < nstep; ++t) {
+      rec_ = &A.tape[t];
+      compute();
+      rec_ = nullptr;
+      if (status_ != PBD_OK) return status_;
+    }
+    for (uint32_t t = nstep; t-- > 0;)
+      if (hip_check(launch_adjoint_step(stream_, dev_, A.iters, A.stab, A.tape[t], W), "adjoint step"))
+        return status_;
+  }
+  if (hip_check(launch_adjoint_reduce(stream_, n_, W.acc, A.red), "adjoint reduce")) return status_;
+  double red[2] = {0.0, 0.0};
+  uint32_t flag = 0;
+  e = hipMemcpyAsync(red, A.red, sizeof(red), hipMemcpyDeviceToHost, stream_);
+  if (e == hipSuccess) e = hipMemcpyAsync(&flag, W.flag, 4, hipMemcpyDeviceToHost, stream_);
+  if (e == hipSuccess && gv0) e = hipMemcpyAsync(gv0, W.GV, n * 16, hipMemcpyDeviceToHost, stream_);
+  if (e == hipSuccess && gx0) e = hipMemcpyAsync(gx0, W.GX, n * 16, hipMemcpyDeviceToHost, stream_);
+  // leave the context as pbd_adj_forward left it
+  const float4* fs = A.final_state;
+  if (e == hipSuccess) e = hipMemcpyAsync(B_.X, fs, n * 16, hipMemcpyDeviceToDevice, stream_);
+  if (e == hipSuccess) e = hipMemcpyAsync(B_.P, fs + n, n * 16, hipMemcpyDeviceToDevice, stream_);
+  if (e == hipSuccess) e = hipMemcpyAsync(B_.V, fs + 2 * n, n * 16, hipMemcpyDeviceToDevice, stream_);
+  if (e == hipSuccess) e = hipMemcpyAsync(B_.C, fs + 3 * n, n * 16, hipMemcpyDeviceToDevice, stream_);
+  if (e == hipSuccess) e = hipMemcpyAsync(B_.F, fs + 4 * n, n * 16, hipMemcpyDeviceToDevice, stream_);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  if (hip_check(e, "adjoint: results")) return status_;
+  if (grho0) *grho0 = (float)red[0];
+  if (geps) *geps = (float)red[1];
+  if (flag)
+    return fail(PBD_ERR_UNSUPPORTED,
+                "adjoint: a neighbour list reached maxNeighboursPerParticle (the capped relation is not symmetric)");
+  return PBD_OK;
+}
+
+}  // namespace pbd

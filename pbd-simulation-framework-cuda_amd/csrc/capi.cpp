@@ -153,6 +153,22 @@ int pbd_step_graph(pbd_ctx* ctx, uint32_t frames) {
   PBD_TRY(ctx, { return guard_status(ctx, ctx->fluid.step(frames, true)); })
 }
 
+int pbd_adj_forward(pbd_ctx* ctx, uint32_t frames, uint32_t k_ckpt) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, { return ctx->fluid.adj_forward(frames, k_ckpt); })
+}
+
+int pbd_adj_ids(pbd_ctx* ctx, uint32_t* initial_index, size_t count) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, { return ctx->fluid.adj_ids(initial_index, count); })
+}
+
+int pbd_adj_backward(pbd_ctx* ctx, const float* dL_dx, const float* dL_dv, float* dL_dv0, float* dL_drho0,
+                     float* dL_deps, float* dL_dx0) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, { return ctx->fluid.adj_backward(dL_dx, dL_dv, dL_dv0, dL_drho0, dL_deps, dL_dx0); })
+}
+
 int pbd_sync(pbd_ctx* ctx) {
   if (!ctx) return PBD_ERR_INVALID;
   PBD_TRY(ctx, { return ctx->fluid.sync(); })

@@ -12,6 +12,7 @@ const char* const kStageNames[ST_NUM] = {"predict_keys", "cell_reset", "sort",  
 
 Fluid::~Fluid() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  adj_release();
   invalidate_graph();
   for (auto& p : pending_) {
     (void)hipEventDestroy(p.a);
@@ -335,6 +336,11 @@ int Fluid::enqueue_substep() {
     tgrid = (std::min(ntiles, P.n) + kTileGridMultiple - 1u) / kTileGridMultiple * kTileGridMultiple;
   }
   const TileList TL{tile_list_, tile_count_};
+  const size_t n16 = (size_t)P.n * 16, n4 = (size_t)P.n * 4;
+  if (rec_) {  // the step's input state, for the reverse of the predict stage
+    if (tape_copy(rec_->Xin, B_.X, n16) || tape_copy(rec_->Vin, B_.V, n16) || tape_copy(rec_->Fin, B_.F, n16))
+      return status_;
+  }
   // 1-2. applyForces + initializeCellIds (Fluid.cu:21-25)
   PBD_LAUNCH(ST_PREDICT, launch_predict_keys(st, tpb, P, B_));
   // resetCellInfo (Fluid.cu:31): empty = {UINT_MAX, N}
@@ -348,6 +354,12 @@ int Fluid::enqueue_substep() {
                                         tile_flag_));
   if (tiles) PBD_LAUNCH(ST_TILES, launch_tile_compact(st, ntiles, tile_flag_, tile_bcount_, tile_list_, tile_count_));
   const uint32_t* keys = keys_sorted_;
+  if (rec_) {
+    const size_t tb = (size_t)B_.table_size * 4;
+    if (tape_copy(rec_->Q, B_.Q[0], n16) || tape_copy(rec_->keys, keys_sorted_, n4) ||
+        tape_copy(rec_->perm, ids_sorted_, n4) || tape_copy(rec_->cs, B_.cstart, tb) || tape_copy(rec_->ce, B_.cend, tb))
+      return status_;
+  }
 
   // 6-7. solver loop (Fluid.cu:49-80); contacts/neighbours are walked inside the passes
   const bool lambda_on = g & G_LAMBDA, delta_on = g & G_DELTA, apply_on = g & G_APPLY_DELTA;
@@ -371,6 +383,7 @@ int Fluid::enqueue_substep() {
         PBD_LAUNCH(ST_LAMBDA, launch_lambda(st, tpb, P, B_, B_.Q[cur], keys, !counted));
       }
       counted = true;
+      if (rec_ && tape_copy(rec_->LAM + (size_t)it * P.n, B_.LAM, n4)) return status_;
     }
     // the Q-writing kernels of this iteration
     struct K {
@@ -419,6 +432,7 @@ int Fluid::enqueue_substep() {
       }
       cur = 1 - cur;
     }
+    if (rec_ && tape_copy(rec_->Q + (size_t)(it + 1) * P.n, B_.Q[cur], n16)) return status_;
   }
   if (!counted) PBD_LAUNCH(ST_COUNT, launch_count(st, tpb, P, B_, keys));
   // 8. updatePositions (Fluid.cu:82-84) when not fused into the last solver pass
@@ -435,6 +449,7 @@ int Fluid::enqueue_substep() {
       PBD_LAUNCH(ST_OMEGA, launch_omega(st, tpb, P, B_, B_.Q[cur], keys));
     }
   }
+  if (rec_ && (tape_copy(rec_->U, B_.VS, n16) || tape_copy(rec_->OM, B_.OM, n16))) return status_;
   // 10-11. computeVorticity + computeViscosity (Fluid.cu:90-96) and the output
   uint32_t ff = 0;
   if (g & G_VORTICITY) ff |= FF_VORT;
@@ -474,32 +489,44 @@ int Fluid::step(uint32_t frames, bool graph) {
       }
     return PBD_OK;
   }
-  if (!graph_valid_) {
-    if (!dev_valid_) {
-      const char* why = nullptr;
-      if (derive_device_params(params_, n_, dev_, &why) != PBD_OK) return fail(PBD_ERR_INVALID, why);
-      apply_scenes();
-      dev_valid_ = true;
-    }
-    if (hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
-      return status_;
-    int rc = enqueue_substep();
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(stream_, &g);
-    if (rc != PBD_OK) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
-    }
-    if (hip_check(e, "hipStreamEndCapture")) return status_;
-    graph_ = g;
-    if (hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate"))
-      return status_;
-    graph_valid_ = true;
-  }
+  if (int rc = ensure_graph()) return rc;
   for (uint32_t f = 0; f < frames; ++f)
     for (uint32_t s = 0; s < sub; ++s)
       if (hip_check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch")) return status_;
   return PBD_OK;
+}
+
+// Capture one substep into graph_exec_ (once per parameter / particle-count change).
+int Fluid::ensure_graph() {
+  if (graph_valid_) return PBD_OK;
+  if (!dev_valid_) {
+    const char* why = nullptr;
+    if (derive_device_params(params_, n_, dev_, &why) != PBD_OK) return fail(PBD_ERR_INVALID, why);
+    apply_scenes();
+    dev_valid_ = true;
+  }
+  if (hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+    return status_;
+  const AdjTape* rec = rec_;
+  rec_ = nullptr;  // a graph never records
+  int rc = enqueue_substep();
+  rec_ = rec;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(stream_, &g);
+  if (rc != PBD_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (hip_check(e, "hipStreamEndCapture")) return status_;
+  graph_ = g;
+  if (hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate"))
+    return status_;
+  graph_valid_ = true;
+  return PBD_OK;
+}
+
+int Fluid::tape_copy(void* dst, const void* src, size_t bytes) {
+  return hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_), "adjoint record copy");
 }
 
 int Fluid::sync() {

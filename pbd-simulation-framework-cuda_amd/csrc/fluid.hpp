@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "adjoint.hpp"
 #include "kernels.hpp"
 #include "params.hpp"
 #include "pbd/pbd.h"
@@ -68,6 +69,12 @@ class Fluid : public Cuda_Computable {
   void set_scenes(const SceneParams* table, uint32_t n, uint32_t slot_log2);
   hipStream_t stream() const { return stream_; }
 
+  // Reverse mode (adjoint_host.cpp): a checkpointed forward of `frames` frames,
+  // then the gradients of a loss of the final state.
+  int adj_forward(uint32_t frames, uint32_t k_ckpt);
+  int adj_ids(uint32_t* dst, size_t count);
+  int adj_backward(const float* gx, const float* gv, float* gv0, float* grho0, float* geps, float* gx0);
+
   int status() const { return status_; }
   const std::string& error() const { return error_; }
   void clear_status() {
@@ -82,6 +89,10 @@ class Fluid : public Cuda_Computable {
   int ensure_batch_buffers();
   void apply_scenes();
   int enqueue_substep();
+  int ensure_graph();
+  int adj_check();
+  void adj_release();
+  int tape_copy(void* dst, const void* src, size_t bytes);
   void stage_begin(int stage);
   void stage_end(int stage);
   int collect_timings();
@@ -109,6 +120,12 @@ class Fluid : public Cuda_Computable {
   uint32_t batch_words_ = 0;
   const SceneParams* scene_table_ = nullptr;  // batched scenes
   uint32_t scene_count_ = 1, slot_log2_ = 10;
+
+  // reverse mode: checkpoints, the record of one segment and the work
+  // buffers (allocated by adj_forward); rec_ is the step record being written
+  struct AdjState;
+  AdjState* adj_ = nullptr;
+  const AdjTape* rec_ = nullptr;
 
   // graph of one substep
   hipGraph_t graph_ = nullptr;

@@ -169,6 +169,10 @@ _SIGS = {
     "pbd_reset_timings": (C.c_int, [C.c_void_p]),
     "pbd_get_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint32, C.POINTER(C.c_uint32)]),
     "pbd_reset_counters": (C.c_int, [C.c_void_p]),
+    "pbd_adj_forward": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "pbd_adj_ids": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "pbd_adj_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float), C.c_void_p]),
     "pbd_last_error": (C.c_char_p, [C.c_void_p]),
     "pbd_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
     "pbd_batch_step": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -402,6 +406,43 @@ class Context:
 
     def reset_counters(self) -> None:
         self._check(lib().pbd_reset_counters(self._ptr), "pbd_reset_counters")
+
+    # ---- reverse mode
+    def adj_forward(self, frames: int, k_ckpt: int) -> None:
+        """Checkpointed forward of `frames` frames (pbd_adj_forward)."""
+        self._check(lib().pbd_adj_forward(self._ptr, int(frames), int(k_ckpt)), "pbd_adj_forward")
+
+    def adj_ids(self) -> np.ndarray:
+        """Per particle of the final order: its index in the initial order."""
+        n = self.n
+        out = np.zeros(n, np.uint32)
+        self._check(lib().pbd_adj_ids(self._ptr, out.ctypes.data, n), "pbd_adj_ids")
+        return out
+
+    def adj_backward(self, dL_dx=None, dL_dv=None) -> dict:
+        """Gradients of a loss with adjoints dL_dx, dL_dv ((n, 3) or (n, 4), final
+        order) of the final state: dict(v0, x0: (n, 3) in the initial order,
+        rho0, eps: floats)."""
+        n = self.n
+
+        def f4(a):
+            if a is None:
+                return None
+            a = np.asarray(a, np.float32).reshape(n, -1)
+            out = np.zeros((n, 4), np.float32)
+            out[:, :3] = a[:, :3]
+            return out
+
+        gx, gv = f4(dL_dx), f4(dL_dv)
+        gv0 = np.zeros((n, 4), np.float32)
+        gx0 = np.zeros((n, 4), np.float32)
+        gr = C.c_float(0.0)
+        ge = C.c_float(0.0)
+        rc = lib().pbd_adj_backward(self._ptr, None if gx is None else gx.ctypes.data,
+                                    None if gv is None else gv.ctypes.data, gv0.ctypes.data, C.byref(gr),
+                                    C.byref(ge), gx0.ctypes.data)
+        self._check(rc, "pbd_adj_backward")
+        return dict(v0=gv0[:, :3].copy(), x0=gx0[:, :3].copy(), rho0=gr.value, eps=ge.value)
 
 
 class Batch:

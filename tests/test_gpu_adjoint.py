@@ -1,0 +1,202 @@
+"""GPU reverse mode (pbd_adj_forward / pbd_adj_backward through the C-ABI)
+against the FP64 oracle: its reverse mode on the same FP32 initial state, FP64
+central finite differences of its forward map, and closed forms.
+
+Tolerances: 1e-3 relative (FP32 adjoint of an FP32 forward against FP64).
+Horizons stay where the finite differences converge: the step's tangent map
+grows about x2 per step under the particle-collision pass."""
+import numpy as np
+import pytest
+
+from conftest import small_dam
+
+pytestmark = pytest.mark.gpu
+
+
+def flush(msg):
+    print(msg, flush=True)
+
+
+def _ctx(F, p, x0, v0):
+    q = p.copy()
+    q.box_spawn_in_the_beginning = 0
+    n = x0.shape[0]
+    ctx = F.Context(q, n)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = x0
+    pos[:, 3] = 1.0
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, :3] = v0
+    ctx.add_particles(pos, vel)
+    return ctx
+
+
+def _scene(F, side, seed=1, vstd=0.1):
+    p = small_dam(F, side)
+    pos, _, _ = F.generate_box(p)
+    x0 = pos[:, :3].astype(np.float32)
+    rng = np.random.default_rng(seed)
+    v0 = rng.normal(0.0, vstd, x0.shape).astype(np.float32)
+    return p, x0, v0, rng
+
+
+def _gpu_grad(F, p, x0, v0, frames, k, target):
+    ctx = _ctx(F, p, x0, v0)
+    ctx.adj_forward(frames, k)
+    ids = ctx.adj_ids().astype(np.int64)
+    xT = ctx.download("positions")[:, :3].astype(np.float64)
+    n = x0.shape[0]
+    gx = 2.0 * (xT - target[ids]) / n
+    g = ctx.adj_backward(gx, None)
+    ctx.close()
+    return g, ids, xT
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def test_gpu_adjoint_free_flight(F, O):
+    """Isolated particles: dL/dv0 = N dt dL/dx_T + dL/dv_T, dL/dx0 = dL/dx_T."""
+    p = small_dam(F, 4)
+    p.box_spacing = 0.3
+    p.box_origin[0], p.box_origin[1], p.box_origin[2] = 0.6, 0.5, 0.3
+    pos, _, _ = F.generate_box(p)
+    x0 = pos[:, :3].astype(np.float32)
+    n = x0.shape[0]
+    rng = np.random.default_rng(3)
+    v0 = rng.normal(0.0, 0.05, x0.shape).astype(np.float32)
+    flush("free flight: n=%d frames=2" % n)
+    ctx = _ctx(F, p, x0, v0)
+    ctx.adj_forward(2, 1)
+    ids = ctx.adj_ids().astype(np.int64)
+    gx = rng.normal(size=(n, 3)).astype(np.float32)
+    gv = rng.normal(size=(n, 3)).astype(np.float32)
+    g = ctx.adj_backward(gx, gv)
+    inv = np.empty(n, np.int64)
+    inv[ids] = np.arange(n)
+    nsub = 2 * p.substeps
+    dt = np.float32(p.delta_t) / np.float32(p.substeps)
+    np.testing.assert_allclose(g["v0"], nsub * dt * gx[inv] + gv[inv], rtol=1e-5, atol=1e-6)
+    # dL/dx0 is the sum of the predicted-position adjoint (dL/dx_T + dL/dv_T / dt)
+    # and the position adjoint (-dL/dv_T / dt): FP32 keeps it to ~eps32 |dL/dv_T| / dt
+    np.testing.assert_allclose(g["x0"], gx[inv], rtol=1e-5, atol=8 * 6e-8 * np.abs(gv).max() / dt)
+    assert g["rho0"] == 0.0 and g["eps"] == 0.0
+
+
+@pytest.mark.parametrize("side,frames", [(8, 1), (12, 1)])
+def test_gpu_adjoint_matches_oracle_adjoint(F, O, side, frames):
+    p, x0, v0, rng = _scene(F, side)
+    n = x0.shape[0]
+    nsub = frames * p.substeps
+    flush(f"adjoint vs oracle adjoint: n={n} steps={nsub}")
+    xf, _, _, orig = O.forward64(p, x0.astype(np.float64), v0.astype(np.float64), nsub=nsub)
+    target = xf[np.argsort(orig)] + rng.normal(0.0, 0.01, x0.shape)
+    g, ids, xT = _gpu_grad(F, p, x0, v0, frames, 1, target)
+    X, _, _, og = O.forward64(p, x0.astype(np.float64), v0.astype(np.float64), nsub=nsub)
+    assert np.abs(X[np.argsort(og)] - xT[np.argsort(ids)]).max() < 1e-4
+    gxo = 2.0 * (X - target[og]) / n
+    go = O.gradient64(p, x0.astype(np.float64), v0.astype(np.float64), gxo, np.zeros_like(gxo), nsub=nsub)
+    for k in ("v0", "x0"):
+        err = np.linalg.norm(g[k] - go[k]) / np.linalg.norm(go[k])
+        flush(f"  {k}: relative L2 error {err:.2e}")
+        assert err < 1e-3
+    for k in ("rho0", "eps"):
+        flush(f"  {k}: gpu {g[k]:.6e} oracle {go[k]:.6e}")
+        assert _rel(g[k], go[k]) < 1e-3
+
+
+def test_gpu_adjoint_matches_central_differences(F, O):
+    """GPU gradient against FP64 central differences of the oracle's forward
+    over 2 frames (4 steps), 8 random velocity directions, rho0 and eps."""
+    p, x0, v0, rng = _scene(F, 8)
+    n = x0.shape[0]
+    frames = 2
+    nsub = frames * p.substeps
+    x64, v64 = x0.astype(np.float64), v0.astype(np.float64)
+    xf, _, _, orig = O.forward64(p, x64, v64, nsub=nsub)
+    target = xf[np.argsort(orig)] + rng.normal(0.0, 0.01, x0.shape)
+
+    def loss(v=v64, rho0=0.0, eps=-1.0):
+        X, _, _, og = O.forward64(p, x64, v, nsub=nsub, rho0=rho0, eps=eps)
+        return np.sum((X - target[og]) ** 2) / n
+
+    flush(f"adjoint vs central differences: n={n} steps={nsub}")
+    g, _, _ = _gpu_grad(F, p, x0, v0, frames, 1, target)
+    h = 1e-7
+    for k in range(8):
+        u = rng.normal(size=x0.shape)
+        fd = (loss(v=v64 + h * u) - loss(v=v64 - h * u)) / (2 * h)
+        e = _rel(np.sum(g["v0"] * u), fd)
+        flush(f"  direction {k}: fd {fd:.6e} relative error {e:.2e}")
+        assert e < 1e-3
+    d = O.derive(p)
+    hr = 1e-6
+    fd = (loss(rho0=d.rho0 * (1 + hr)) - loss(rho0=d.rho0 * (1 - hr))) / (2 * hr * d.rho0)
+    flush(f"  rho0: gpu {g['rho0']:.6e} fd {fd:.6e}")
+    assert _rel(g["rho0"], fd) < 1e-3
+    fd = (loss(eps=d.eps * (1 + hr)) - loss(eps=d.eps * (1 - hr))) / (2 * hr * d.eps)
+    flush(f"  eps: gpu {g['eps']:.6e} fd {fd:.6e}")
+    assert _rel(g["eps"], fd) < 1e-3
+
+
+def test_gpu_adjoint_checkpoint_interval_is_exact(F, O):
+    """Recomputing segments from checkpoints replays the forward bit for bit:
+    the gradient does not depend on the checkpoint interval."""
+    p, x0, v0, rng = _scene(F, 8)
+    target = x0.astype(np.float64) + rng.normal(0.0, 0.05, x0.shape)
+    flush("checkpoint intervals 1, 3, 4 over 4 frames")
+    g1, ids1, x1 = _gpu_grad(F, p, x0, v0, 4, 1, target)
+    for k in (3, 4):
+        gk, idsk, xk = _gpu_grad(F, p, x0, v0, 4, k, target)
+        assert np.array_equal(ids1, idsk) and np.array_equal(x1, xk)
+        assert np.array_equal(g1["v0"], gk["v0"]) and np.array_equal(g1["x0"], gk["x0"])
+        assert g1["rho0"] == gk["rho0"] and g1["eps"] == gk["eps"]
+
+
+def test_gpu_adjoint_torch_harness(F, O):
+    """torch.autograd.Function over the C-ABI: loss.backward() gives the
+    gradients of the direct calls."""
+    import torch
+
+    from pbdfluid.autograd import rollout
+
+    p, x0, v0, rng = _scene(F, 8)
+    rho0 = torch.tensor(float(np.float32(O.derive(p).rho0)), requires_grad=True)
+    eps = torch.tensor(float(p.relaxation), requires_grad=True)
+    v = torch.from_numpy(v0.copy()).requires_grad_(True)
+    x = torch.from_numpy(x0.copy()).requires_grad_(True)
+    target = torch.from_numpy(x0 + rng.normal(0.0, 0.05, x0.shape).astype(np.float32))
+    flush("torch harness: rollout of 2 frames")
+    xT, vT = rollout(v, rho0, eps, x, p, 2, 1)
+    loss = ((xT - target) ** 2).sum() / x0.shape[0] + 0.1 * (vT ** 2).sum() / x0.shape[0]
+    loss.backward()
+    # the same gradient through the context API
+    q = p.copy()
+    q.rest_density = float(rho0.detach())
+    q.rest_density_from_lattice = 0
+    ctx = _ctx(F, q, x0, v0)
+    ctx.adj_forward(2, 1)
+    ids = ctx.adj_ids().astype(np.int64)
+    xf = ctx.download("positions")[:, :3]
+    vf = ctx.download("velocities")[:, :3]
+    n = x0.shape[0]
+    g = ctx.adj_backward(2.0 * (xf - target.numpy()[ids]) / n, 0.2 * vf / n)
+    np.testing.assert_allclose(v.grad.numpy(), g["v0"], rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(x.grad.numpy(), g["x0"], rtol=1e-5, atol=1e-9)
+    assert _rel(float(rho0.grad), g["rho0"]) < 1e-5 and _rel(float(eps.grad), g["eps"]) < 1e-5
+
+
+def test_gpu_adjoint_errors(F, O):
+    p, x0, v0, _ = _scene(F, 4)
+    ctx = _ctx(F, p, x0, v0)
+    with pytest.raises(F.PbdError):
+        ctx.adj_backward(np.zeros((x0.shape[0], 3)), None)  # no forward yet
+    ctx.close()
+    q = p.copy()
+    q.kernel_variant = F.PBD_KERNEL_REFERENCE
+    ctx = _ctx(F, q, x0, v0)
+    with pytest.raises(F.PbdError) as e:
+        ctx.adj_forward(1, 1)
+    assert e.value.code == 6  # PBD_ERR_UNSUPPORTED
+    ctx.close()
