@@ -138,6 +138,95 @@ def batched_vs_plain(F, n_scenes, steps, warmup):
     }
 
 
+def adjoint_leg(F, frames, k_ckpt, seed=5):
+    """adj/fwd on the 65,536-particle adjoint scene: the checkpointed forward
+    (pbd_adj_forward), the segment recomputation and the reverse passes
+    (pbd_adj_backward) against the plain graph-launched forward of the same
+    frames from the same state.  Loss: mean squared distance of the final
+    positions to the targets of a seeded run with rest density +5% and
+    initial velocities ~ N(0, 0.1^2); the differentiated run starts at rest."""
+    p = F.load_params(F.config_path("adjoint_64k"))
+    pos, _, _ = F.generate_box(p)
+    n = pos.shape[0]
+    rho0 = float(np.float32(F.lattice_rest_density(p)))
+    rng = np.random.default_rng(seed)
+    v_target = rng.normal(0.0, 0.1, (n, 3)).astype(np.float32)
+
+    def context(rho, v):
+        q = p.copy()
+        q.box_spawn_in_the_beginning = 0
+        q.rest_density = rho
+        q.rest_density_from_lattice = 0
+        c = F.Context(q, n)
+        vel = np.zeros((n, 4), np.float32)
+        vel[:, :3] = v
+        c.add_particles(pos, vel)
+        c.step(0, graph=True)  # capture the step graph outside the timed regions
+        return c
+
+    def targets(fr):
+        c = context(1.05 * rho0, v_target)
+        c.adj_forward(fr, fr)
+        ids = c.adj_ids().astype(np.int64)
+        t = np.zeros((n, 3))
+        t[ids] = c.download("positions")[:, :3]
+        c.close()
+        return t
+
+    def gradient(fr, k, target, timed):
+        c = context(rho0, np.zeros((n, 3), np.float32))
+        c.sync()
+        t0 = time.perf_counter()
+        c.adj_forward(fr, k)
+        c.sync()
+        t_fwd = time.perf_counter() - t0
+        ids = c.adj_ids().astype(np.int64)
+        xT = c.download("positions")[:, :3].astype(np.float64)
+        gx = 2.0 * (xT - target[ids]) / n
+        t0 = time.perf_counter()
+        g = c.adj_backward(gx, None)
+        t_bwd = time.perf_counter() - t0
+        c.close()
+        return g, t_fwd, t_bwd, float(np.sum((xT - target[ids]) ** 2) / n), float(np.linalg.norm(gx))
+
+    # plain forward of the same frames
+    c = context(rho0, np.zeros((n, 3), np.float32))
+    c.sync()
+    t0 = time.perf_counter()
+    c.step(frames, graph=True)
+    c.sync()
+    t_plain = time.perf_counter() - t0
+    c.close()
+    target = targets(frames)
+    g, t_af, t_ab, loss, gxn = gradient(frames, k_ckpt, target, True)
+    growth = {}
+    for fr in (1, 5, 10, 30):
+        if fr < frames:
+            gs, _, _, _, gsn = gradient(fr, min(k_ckpt, fr), targets(fr), False)
+            growth[str(fr)] = float(np.linalg.norm(gs["v0"]) / gsn) if gsn > 0 else None
+    gn = float(np.linalg.norm(g["v0"]))
+    growth[str(frames)] = gn / gxn if gxn > 0 else None
+    return {
+        "particles": n,
+        "frames": frames,
+        "substeps_per_frame": p.substeps,
+        "checkpoint_every_frames": k_ckpt,
+        "plain_forward_s": t_plain,
+        "adj_forward_s": t_af,
+        "adj_backward_s": t_ab,
+        "adj_over_fwd": (t_af + t_ab) / t_plain,
+        "loss": loss,
+        "dL_dv0_norm": gn,
+        "dL_dv0_finite": bool(np.isfinite(g["v0"]).all()),
+        "dL_drho0": g["rho0"],
+        "dL_deps": g["eps"],
+        "gradient_growth": growth,
+        "workload": "adjoint_64k: 64x32x32 lattice (65,536 particles) in a 6.4x1.6x1.6 box, the 32k dam-break "
+                    "parameters; loss = mean |x_T - target|^2, targets from rest density +5% and v0 ~ N(0, 0.1^2); "
+                    "gradient_growth = |dL/dv0| / |dL/dx_T| per horizon in frames",
+    }
+
+
 def copy_calibration():
     """Three copies of 256 MiB between device buffers (torch's copy kernel)."""
     import torch
@@ -187,6 +276,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-frames", type=int, default=60)
+    ap.add_argument("--adjoint-checkpoint", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy (the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters to HBM bytes)")
@@ -248,6 +340,7 @@ def main():
         v["algorithmic_GBps"] = (b / (v["us_per_launch"] * 1e-6) / 1e9) if v["us_per_launch"] else 0.0
     traffic = measured_traffic(dom)
     batched = None if args.no_batched else batched_vs_plain(F, args.scenes, args.batched_steps, 3)
+    adjoint = None if args.no_adjoint else adjoint_leg(F, args.adjoint_frames, args.adjoint_checkpoint)
 
     line = {
         "metric": METRIC,
@@ -286,6 +379,7 @@ def main():
         "nonfinite_particles": bad,
         "neighbour_lists": counters,
         "batched": batched,
+        "adjoint": adjoint,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, args.cpu_steps)

@@ -20,7 +20,8 @@
 // kernels flag it otherwise), so the adjoint of a pair sum is a gather over
 // the same neighbours combining the i-side and j-side adjoints, e.g.
 //   d/dx_i sum_k a_k . sum_j G_kj = sum_j H_ij (a_i - a_j).
-// Passes per step (last stage first), each one kernel over the particles:
+// Passes per step (last stage first), each one kernel over the particles
+// (kLanes lanes per particle for the neighbour passes):
 //   A  XSPH (self-adjoint gather), eta and the vorticity-force adjoint
 //   B  |omega| adjoint through eta; position terms of eta
 //   C  omega adjoint, the update, the clamp mask of the last iteration
@@ -110,6 +111,64 @@ __device__ __forceinline__ float4 masked(float3 v, uint32_t m) {
 
 __device__ __forceinline__ bool shells_on(const DevParams& P) { return (P.gates & G_FIND_NEIGHBOURS) != 0; }
 
+// Lanes per particle: the neighbour candidates of one particle are dealt to
+// kLanes consecutive lanes (candidate j to lane j mod kLanes within each
+// cell), so that a step of a few ten thousand particles still fills the CUs
+// with waves; the per-lane sums are combined by butterfly shuffles.
+constexpr uint32_t kLanes = 8;
+
+__device__ __forceinline__ float lane_sum(float v) {
+#pragma unroll
+  for (uint32_t o = kLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, (int)o);
+  return v;
+}
+__device__ __forceinline__ float3 lane_sum(float3 v) { return make_float3(lane_sum(v.x), lane_sum(v.y), lane_sum(v.z)); }
+
+// The stencil of visit_neighbours (27 cells in (x, y, z) order -1..1, then the
+// shells up to kernelWidthNeighbours when findNeighbours is on), each lane
+// taking its share of every cell.  body(j, bx, by, bz, contact) as in
+// visit_neighbours; returns the particle's neighbour count (all lanes).
+template <class Body>
+__device__ __forceinline__ uint32_t adj_visit(const DevParams& P, uint32_t s, uint32_t lane, const CellFrame& f,
+                                              float qx, float qy, float qz, const uint32_t* __restrict__ cs,
+                                              const uint32_t* __restrict__ ce, Body body) {
+  uint32_t cnt = 0;
+  const int kw = shells_on(P) ? (P.kw > 1 ? P.kw : 1) : 1;
+  for (int sh = 1; sh <= kw; ++sh) {
+    for (int dx = -sh; dx <= sh; ++dx) {
+      const uint32_t nx = f.ix + (uint32_t)dx;
+      if (nx >= P.dims[0]) continue;
+      const uint32_t kx = dilated_add(f.kx, dx, kMortonX, 2);
+      const float bx = qx + (f.cx - cell_corner(P.origin[0], P.cell, nx));
+      const bool ex = (dx == -sh || dx == sh);
+      for (int dy = -sh; dy <= sh; ++dy) {
+        const uint32_t ny = f.iy + (uint32_t)dy;
+        if (ny >= P.dims[1]) continue;
+        const uint32_t ky = dilated_add(f.ky, dy, kMortonY, 1);
+        const float by = qy + (f.cy - cell_corner(P.origin[1], P.cell, ny));
+        const bool ey = ex || (dy == -sh || dy == sh);
+        for (int dz = -sh; dz <= sh; ++dz) {
+          if (sh > 1 && !ey && dz != -sh && dz != sh) continue;  // shell sh: its surface only
+          const uint32_t nz = f.iz + (uint32_t)dz;
+          if (nz >= P.dims[2]) continue;
+          const uint32_t kz = dilated_add(f.kz, dz, kMortonZ, 0);
+          const float bz = qz + (f.cz - cell_corner(P.origin[2], P.cell, nz));
+          const uint32_t k = kx | ky | kz;
+          const uint32_t st = cs[k];
+          if (st == 0xFFFFFFFFu) continue;
+          const uint32_t en = ce[k];
+          cnt += en - st - ((s >= st && s < en) ? 1u : 0u);
+          for (uint32_t j = st + ((lane - st) & (kLanes - 1)); j < en; j += kLanes) {
+            if (j == s) continue;
+            body(j, bx, by, bz, sh == 1);
+          }
+        }
+      }
+    }
+  }
+  return cnt;
+}
+
 // ---------------------------------------------------------------- A
 __global__ void __launch_bounds__(kAdjThreads)
 k_adj_a(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
@@ -117,16 +176,15 @@ k_adj_a(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ 
         const float4* __restrict__ GV, const float4* __restrict__ GF, float4* __restrict__ Ub,
         float4* __restrict__ Xp, float4* __restrict__ Eb, float4* __restrict__ Ob, double2* __restrict__ acc,
         uint32_t* __restrict__ flag) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t s = t / kLanes, lane = t % kLanes;
   if (s >= P.n) return;
   const float4 qi = Q[s], oi4 = OM[s];
   const float3 ui = f3(U[s]), gvi = f3(GV[s]), oi = f3(oi4);
   const CellFrame f = make_frame(P, keys[s]);
   const float cv = P.c_visc * P.vol;
-  float3 ub = gvi, xb = make_float3(0.f, 0.f, 0.f), eta = xb, xs = xb;
-  uint32_t nc = 0;
-  const uint32_t cnt = visit_neighbours(
-      P, s, f, qi.x, qi.y, qi.z, true, shells_on(P), cs, ce, nc, [&](uint32_t j, float bx, float by, float bz, bool) {
+  float3 ub = make_float3(0.f, 0.f, 0.f), xb = ub, eta = ub, xs = ub;
+  const uint32_t cnt = adj_visit(P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
         const float4 qj = Q[j];
         const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
         const AdjPair q = adj_pair(P, dot(r, r));
@@ -136,6 +194,11 @@ k_adj_a(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ 
         xb += (-cv * dot(du, dg) * q.wgs) * r;
         eta += ((OM[j].w - oi4.w) * q.gs) * r;
       });
+  ub = gvi + lane_sum(ub);
+  xb = lane_sum(xb);
+  eta = lane_sum(eta);
+  xs = lane_sum(xs);
+  if (lane) return;
   if (cnt >= P.cap) flag[0] = 1u;
   acc[s].x += (double)(-cv * dot(gvi, xs) * P.inv_rho0);
   const float el = sqrtf(dot(eta, eta));
@@ -155,16 +218,15 @@ __global__ void __launch_bounds__(kAdjThreads)
 k_adj_b(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
         const uint32_t* __restrict__ ce, const float4* __restrict__ OM, const float4* __restrict__ Eb,
         float4* __restrict__ Ob, float4* __restrict__ Xp) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t s = t / kLanes, lane = t % kLanes;
   if (s >= P.n) return;
   const float4 qi = Q[s], oi4 = OM[s];
   const float3 ebi = f3(Eb[s]);
   const CellFrame f = make_frame(P, keys[s]);
   float mb = 0.0f;
   float3 xb = make_float3(0.f, 0.f, 0.f);
-  uint32_t nc = 0;
-  visit_neighbours(P, s, f, qi.x, qi.y, qi.z, true, shells_on(P), cs, ce, nc,
-                   [&](uint32_t j, float bx, float by, float bz, bool) {
+  adj_visit(P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
                      const float4 qj = Q[j];
                      const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
                      const AdjPair q = adj_pair(P, dot(r, r));
@@ -172,6 +234,9 @@ k_adj_b(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ 
                      mb -= q.gs * dot(es, r);
                      xb += (OM[j].w - oi4.w) * hmul(q, r, es);
                    });
+  mb = lane_sum(mb);
+  xb = lane_sum(xb);
+  if (lane) return;
   const float4 x = Xp[s];
   Xp[s] = make_float4(x.x + xb.x, x.y + xb.y, x.z + xb.z, 0.0f);
   float3 ob = f3(Ob[s]);
@@ -186,16 +251,15 @@ k_adj_c(DevParams P, uint32_t stab, const float4* __restrict__ Q, const uint32_t
         const float4* __restrict__ OM, const float4* __restrict__ Ob, const float4* __restrict__ Ub,
         const float4* __restrict__ Xp, const float4* __restrict__ GX, float4* __restrict__ Xb,
         float4* __restrict__ DPb, float4* __restrict__ PUb, double2* __restrict__ acc) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t s = t / kLanes, lane = t % kLanes;
   if (s >= P.n) return;
   const float4 qi = Q[s];
   const uint32_t key = keys[s];
   const float3 ui = f3(U[s]), obi = f3(Ob[s]);
   const CellFrame f = make_frame(P, key);
   float3 ub = make_float3(0.f, 0.f, 0.f), xb = ub;
-  uint32_t nc = 0;
-  visit_neighbours(P, s, f, qi.x, qi.y, qi.z, true, shells_on(P), cs, ce, nc,
-                   [&](uint32_t j, float bx, float by, float bz, bool) {
+  adj_visit(P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
                      const float4 qj = Q[j];
                      const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
                      const AdjPair q = adj_pair(P, dot(r, r));
@@ -203,6 +267,9 @@ k_adj_c(DevParams P, uint32_t stab, const float4* __restrict__ Q, const uint32_t
                      ub += (-P.inv_rho0 * q.gs) * cross(os, r);
                      xb += P.inv_rho0 * hmul(q, r, cross(f3(U[j]) - ui, os));
                    });
+  ub = lane_sum(ub);
+  xb = lane_sum(xb);
+  if (lane) return;
   acc[s].x += (double)(-dot(obi, f3(OM[s])) * P.inv_rho0);
   const float3 u = f3(Ub[s]) + ub;
   const float3 pb = f3(GX[s]) + f3(Xp[s]) + xb + P.inv_dt * u;
@@ -219,7 +286,8 @@ k_adj_r1(DevParams P, uint32_t stab, const float4* __restrict__ Q, const float* 
          const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
          const float4* __restrict__ DPb, const float4* __restrict__ PUb, float4* __restrict__ Pn,
          float4* __restrict__ AB, float* __restrict__ Rb, double2* __restrict__ acc, uint32_t* __restrict__ flag) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t s = t / kLanes, lane = t % kLanes;
   if (s >= P.n) return;
   const float4 qi = Q[s];
   const uint32_t key = keys[s];
@@ -227,11 +295,9 @@ k_adj_r1(DevParams P, uint32_t stab, const float4* __restrict__ Q, const float* 
   const float3 dpi = f3(DPb[s]), pui = f3(PUb[s]);
   const CellFrame f = make_frame(P, key);
   float A = 0.0f, B = 0.0f, lb = 0.0f;
-  float3 a = make_float3(0.f, 0.f, 0.f), dsum = a, pn = dpi;
-  uint32_t nc = 0;
-  const uint32_t cnt = visit_neighbours(
-      P, s, f, qi.x, qi.y, qi.z, true, shells_on(P), cs, ce, nc,
-      [&](uint32_t j, float bx, float by, float bz, bool contact) {
+  float3 a = make_float3(0.f, 0.f, 0.f), dsum = a, pn = a;
+  const uint32_t cnt = adj_visit(
+      P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool contact) {
         const float4 qj = Q[j];
         const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
         const float r2 = dot(r, r);
@@ -255,6 +321,13 @@ k_adj_r1(DevParams P, uint32_t stab, const float4* __restrict__ Q, const float* 
           pn += kc * (phi * pd - rr * r);
         }
       });
+  A = lane_sum(A);
+  B = lane_sum(B);
+  lb = lane_sum(lb);
+  a = lane_sum(a);
+  dsum = lane_sum(dsum);
+  pn = dpi + lane_sum(pn);
+  if (lane) return;
   if (cnt >= P.cap) flag[0] = 1u;
   lb *= P.inv_rho0;
   const float rho = A + P.w0;
@@ -282,17 +355,16 @@ k_adj_r2(DevParams P, uint32_t stab, uint32_t last, const float4* __restrict__ Q
          const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce, const float4* __restrict__ AB,
          const float* __restrict__ Rb, const float4* __restrict__ Pn, const float4* __restrict__ Xb,
          float4* __restrict__ DPb, float4* __restrict__ PUb, float4* __restrict__ Pb) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t s = t / kLanes, lane = t % kLanes;
   if (s >= P.n) return;
   const float4 qi = Q[s];
   const uint32_t key = keys[s];
   const float4 abi = AB[s];
   const float rbi = Rb[s];
   const CellFrame f = make_frame(P, key);
-  float3 pn = f3(Pn[s]);
-  uint32_t nc = 0;
-  visit_neighbours(P, s, f, qi.x, qi.y, qi.z, true, shells_on(P), cs, ce, nc,
-                   [&](uint32_t j, float bx, float by, float bz, bool) {
+  float3 pn = make_float3(0.f, 0.f, 0.f);
+  adj_visit(P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
                      const float4 qj = Q[j];
                      const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
                      const AdjPair q = adj_pair(P, dot(r, r));
@@ -300,6 +372,8 @@ k_adj_r2(DevParams P, uint32_t stab, uint32_t last, const float4* __restrict__ Q
                      pn += ((rbi + Rb[j]) * q.wgs) * r;
                      pn += hmul(q, r, f3(abi) - f3(abj) + (2.0f * (abi.w + abj.w) * q.gs) * r);
                    });
+  pn = f3(Pn[s]) + lane_sum(pn);
+  if (lane) return;
   if (last) {
     Pb[s] = make_float4(pn.x, pn.y, pn.z, 0.0f);
     return;
@@ -389,7 +463,8 @@ hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P, uint32_t iter
   const uint32_t n = P.n;
   if (n == 0) return hipSuccess;
   if (iters == 0) return hipErrorInvalidValue;  // the host rejects steps without solver iterations
-  const dim3 grid((n + kAdjThreads - 1) / kAdjThreads), block(kAdjThreads);
+  const dim3 grid((n * kLanes + kAdjThreads - 1) / kAdjThreads), block(kAdjThreads);
+  const dim3 grid1((n + kAdjThreads - 1) / kAdjThreads);
   const float4* Qf = T.Q + (size_t)iters * n;
   k_adj_a<<<grid, block, 0, st>>>(P, Qf, T.keys, T.cs, T.ce, T.U, T.OM, W.GV, W.GF, W.Ub, W.Xp, W.Eb, W.Ob, W.acc,
                                   W.flag);
@@ -403,7 +478,7 @@ hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P, uint32_t iter
     k_adj_r2<<<grid, block, 0, st>>>(P, stab, it == 0 ? 1u : 0u, Q, T.keys, T.cs, T.ce, W.AB, W.Rb, W.Pn, W.Xb,
                                      W.DPb, W.PUb, W.Pb);
   }
-  k_adj_predict<<<grid, block, 0, st>>>(P, T.perm, T.Xin, T.Vin, T.Fin, W.Pb, W.Xb, W.GX, W.GV, W.GF);
+  k_adj_predict<<<grid1, block, 0, st>>>(P, T.perm, T.Xin, T.Vin, T.Fin, W.Pb, W.Xb, W.GX, W.GV, W.GF);
   return hipGetLastError();
 }
 
