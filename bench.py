@@ -138,7 +138,7 @@ def batched_vs_plain(F, n_scenes, steps, warmup):
     }
 
 
-def adjoint_leg(F, frames, k_ckpt, seed=5):
+def adjoint_leg(F, frames, k_ckpt, seed=5, stabilization=None):
     """adj/fwd on the 65,536-particle adjoint scene: the checkpointed forward
     (pbd_adj_forward), the segment recomputation and the reverse passes
     (pbd_adj_backward) against the plain graph-launched forward of the same
@@ -146,6 +146,8 @@ def adjoint_leg(F, frames, k_ckpt, seed=5):
     positions to the targets of a seeded run with rest density +5% and
     initial velocities ~ N(0, 0.1^2); the differentiated run starts at rest."""
     p = F.load_params(F.config_path("adjoint_64k"))
+    if stabilization is not None:
+        p.stabilization_iterations = stabilization
     pos, _, _ = F.generate_box(p)
     n = pos.shape[0]
     rho0 = float(np.float32(F.lattice_rest_density(p)))
@@ -340,7 +342,14 @@ def main():
         v["algorithmic_GBps"] = (b / (v["us_per_launch"] * 1e-6) / 1e9) if v["us_per_launch"] else 0.0
     traffic = measured_traffic(dom)
     batched = None if args.no_batched else batched_vs_plain(F, args.scenes, args.batched_steps, 3)
-    adjoint = None if args.no_adjoint else adjoint_leg(F, args.adjoint_frames, args.adjoint_checkpoint)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg(F, args.adjoint_frames, args.adjoint_checkpoint)
+        # the same scene without the particle-collision pass, whose tangent map stays within FP32 range
+        nc = adjoint_leg(F, args.adjoint_frames, args.adjoint_checkpoint, stabilization=0)
+        adjoint["without_particle_collisions"] = {k: nc[k] for k in (
+            "plain_forward_s", "adj_forward_s", "adj_backward_s", "adj_over_fwd", "loss", "dL_dv0_norm",
+            "dL_dv0_finite", "dL_drho0", "dL_deps", "gradient_growth")}
 
     line = {
         "metric": METRIC,

@@ -115,7 +115,11 @@ __device__ __forceinline__ bool shells_on(const DevParams& P) { return (P.gates 
 // kLanes consecutive lanes (candidate j to lane j mod kLanes within each
 // cell), so that a step of a few ten thousand particles still fills the CUs
 // with waves; the per-lane sums are combined by butterfly shuffles.
-constexpr uint32_t kLanes = 8;
+#ifndef PBD_ADJ_LANES
+#define PBD_ADJ_LANES 8
+#endif
+constexpr uint32_t kLanes = PBD_ADJ_LANES;
+static_assert(kLanes >= 1 && kLanes <= 64 && (kLanes & (kLanes - 1)) == 0, "lanes per particle: a power of 2");
 
 __device__ __forceinline__ float lane_sum(float v) {
 #pragma unroll

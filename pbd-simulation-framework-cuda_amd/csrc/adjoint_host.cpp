@@ -3,10 +3,12 @@
 //
 // adj_forward runs `frames` frames (frames x substeps computes, one graph
 // launch each) and keeps the state (positions, velocities, external forces) at
-// the start of every k-th frame.  adj_backward walks the segments last first:
-// it restores the segment's checkpoint, recomputes its steps eagerly while
-// every step writes its record (AdjTape), and runs the reverse kernels of the
-// steps last first.  The recomputation replays the same kernels on the same
+// the start of every k-th frame; the steps of the last segment run eagerly and
+// write their records (AdjTape) on the way.  adj_backward walks the segments
+// last first: it restores the segment's checkpoint, recomputes its steps
+// eagerly while every step writes its record (skipped for the last segment
+// while the forward's records are still there), and runs the reverse kernels
+// of the steps last first.  The recomputation replays the same kernels on the same
 // inputs, so it reproduces the forward bit for bit.
 #include <algorithm>
 #include <cstring>
@@ -29,6 +31,7 @@ struct Fluid::AdjState {
   int ids_cur = 0;
   double* red = nullptr;  // 2 doubles
   bool valid = false;
+  bool tail_recorded = false;  // the records hold the last segment (written by the forward)
   pbd_params params{};
 
   ~AdjState() {
@@ -121,15 +124,26 @@ int Fluid::adj_forward(uint32_t frames, uint32_t k_ckpt) {
   };
   if (save(A.ckpt[0])) return status_;
   if (int rc = ensure_graph()) return rc;
+  // The last segment's steps write their records as they run (eagerly, the
+  // same kernels as the graph), so the reverse pass starts without recomputing it.
+  const uint32_t tail = (A.nseg - 1) * A.k;
   for (uint32_t f = 0; f < frames; ++f) {
     for (uint32_t s = 0; s < A.sub; ++s) {
-      if (hip_check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch")) return status_;
+      if (f >= tail) {
+        rec_ = &A.tape[(f - tail) * A.sub + s];
+        compute();
+        rec_ = nullptr;
+        if (status_ != PBD_OK) return status_;
+      } else if (hip_check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch")) {
+        return status_;
+      }
       if (hip_check(launch_gather_ids(stream_, n_, ids_sorted_, A.ids[A.ids_cur], A.ids[1 - A.ids_cur]), "ids"))
         return status_;
       A.ids_cur = 1 - A.ids_cur;
     }
     if ((f + 1) % A.k == 0 && f + 1 < frames && save(A.ckpt[(f + 1) / A.k])) return status_;
   }
+  A.tail_recorded = true;
   float4* fs = A.final_state;
   if (tape_copy(fs, B_.X, n * 16) || tape_copy(fs + n, B_.P, n * 16) || tape_copy(fs + 2 * n, B_.V, n * 16) ||
       tape_copy(fs + 3 * n, B_.C, n * 16) || tape_copy(fs + 4 * n, B_.F, n * 16))
@@ -166,15 +180,18 @@ int Fluid::adj_backward(const float* gx, const float* gv, float* gv0, float* grh
   if (e == hipSuccess) e = hipMemsetAsync(W.flag, 0, 4, stream_);
   if (hip_check(e, "adjoint: upload of the loss adjoints")) return status_;
   for (uint32_t g = A.nseg; g-- > 0;) {
-    const float4* c = A.ckpt[g];
-    if (tape_copy(B_.X, c, n * 16) || tape_copy(B_.V, c + n, n * 16) || tape_copy(B_.F, c + 2 * n, n * 16))
-      return status_;
     const uint32_t nstep = (std::min(A.frames, (g + 1) * A.k) - g * A.k) * A.sub;
-    for (uint32_t t = 0; t < nstep; ++t) {
-      rec_ = &A.tape[t];
-      compute();
-      rec_ = nullptr;
-      if (status_ != PBD_OK) return status_;
+    if (g + 1 < A.nseg || !A.tail_recorded) {
+      const float4* c = A.ckpt[g];
+      if (tape_copy(B_.X, c, n * 16) || tape_copy(B_.V, c + n, n * 16) || tape_copy(B_.F, c + 2 * n, n * 16))
+        return status_;
+      for (uint32_t t = 0; t < nstep; ++t) {
+        rec_ = &A.tape[t];
+        compute();
+        rec_ = nullptr;
+        if (status_ != PBD_OK) return status_;
+      }
+      A.tail_recorded = false;  // the records now hold segment g
     }
     for (uint32_t t = nstep; t-- > 0;)
       if (hip_check(launch_adjoint_step(stream_, dev_, A.iters, A.stab, A.tape[t], W), "adjoint step"))

@@ -154,6 +154,21 @@ def test_gpu_adjoint_checkpoint_interval_is_exact(F, O):
         assert g1["rho0"] == gk["rho0"] and g1["eps"] == gk["eps"]
 
 
+def test_gpu_adjoint_backward_twice(F, O):
+    """A second backward on the same forward (the first overwrote the step
+    records the forward left) gives the same gradient."""
+    p, x0, v0, rng = _scene(F, 8)
+    n = x0.shape[0]
+    for k in (1, 3):
+        ctx = _ctx(F, p, x0, v0)
+        ctx.adj_forward(4, k)
+        gx = rng.normal(size=(n, 3)).astype(np.float32) / n
+        g1 = ctx.adj_backward(gx, None)
+        g2 = ctx.adj_backward(gx, None)
+        assert np.array_equal(g1["v0"], g2["v0"]) and g1["rho0"] == g2["rho0"] and g1["eps"] == g2["eps"]
+        ctx.close()
+
+
 def test_gpu_adjoint_torch_harness(F, O):
     """torch.autograd.Function over the C-ABI: loss.backward() gives the
     gradients of the direct calls."""
