@@ -284,9 +284,16 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy (the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters to HBM bytes)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run the adjoint leg and print its JSON object (for profiling the reverse kernels)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.adjoint_only:
+        import pbdfluid as F
+
+        print(json.dumps(adjoint_leg(F, args.adjoint_frames, args.adjoint_checkpoint)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the step runs on a single "

@@ -288,7 +288,7 @@ int pbd_adj_ids(pbd_ctx* ctx, uint32_t* initial_index, size_t count);
  * Outputs (NULL = not wanted): dL_dv0 and dL_dx0, float4 per particle in the
  * initial order (w = 0); dL_drho0 and dL_deps, scalars.  Each segment of
  * k_ckpt frames is recomputed from its checkpoint while its steps are recorded
- * (about 13 + 20 x solverIterations bytes per particle and step), then the
+ * (about 104 + 20 x solverIterations bytes per particle and step), then the
  * reverse kernels run over the same sorted neighbour structure.  The context
  * is left as pbd_adj_forward left it.  PBD_ERR_UNSUPPORTED if a neighbour list
  * reached maxNeighboursPerParticle (a capped neighbour relation is not
