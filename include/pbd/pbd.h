@@ -45,7 +45,8 @@ enum {
   PBD_ERR_CONFIG = 4,      /* config.txt missing, malformed or key not found    */
   PBD_ERR_NONFINITE = 5,   /* NaN/Inf found by pbd_check_finite                 */
   PBD_ERR_UNSUPPORTED = 6, /* feature not available in this build             */
-  PBD_ERR_NOMEM = 7        /* host or device allocation failed                  */
+  PBD_ERR_NOMEM = 7,       /* host or device allocation failed                  */
+  PBD_ERR_IO = 8           /* a file could not be written                       */
 };
 
 /* ---- kernel variants (Density.kernelVariant) ---------------------------- */
@@ -268,6 +269,23 @@ enum {
 };
 int pbd_get_counters(pbd_ctx* ctx, uint64_t* values, uint32_t cap, uint32_t* count);
 int pbd_reset_counters(pbd_ctx* ctx);
+/* ---- Headless frame output ----
+ * Replaces the GL renderer as the step's output consumer (src/app/Engine.cu:66-72
+ * renders after each Simulation::step).  The renderer draws particle i from
+ * texel (i % width, i / width) of the positions4 / colors4 textures
+ * (src/app/rendering/Particles.cc:266-274, shader/particles/
+ * particleSpriteGeom.vert:35-41); the files keep that index order.
+ *   PBD_OUT_RAW         32-byte header {"PBDF", 1, n, texture width, frame, 0, 0, 0}
+ *                       (u32 LE), then n float4 positions4, then n float4 colors4
+ *   PBD_OUT_PLY_ASCII   PLY, vertex i = particle i: x y z and the renderer's
+ *   PBD_OUT_PLY_BINARY  shade mix((0.95,0.97,0.97), (0,0,1), colors4.x^(1/6)) as uchar rgb */
+enum { PBD_OUT_RAW = 0, PBD_OUT_PLY_ASCII = 1, PBD_OUT_PLY_BINARY = 2 };
+/* The context's current positions4 / colors4 (texture width = Textures.width). */
+int pbd_write_frame(pbd_ctx* ctx, const char* path, int format, uint32_t frame);
+/* The same from host arrays (colors4 NULL = white); errors: pbd_last_error(NULL). */
+int pbd_write_frame_arrays(const char* path, int format, uint32_t frame, uint32_t n, uint32_t texture_width,
+                           const float* positions4, const float* colors4);
+
 /* ---- Reverse mode (no counterpart in the reference) ----
  * Gradients of a loss L(x_T, v_T) of the state after `frames` frames
  * (frames x substeps computes of Fluid::compute, src/app/simulation/Fluid.cu:14-97)

@@ -17,6 +17,11 @@ struct pbd_ctx {
   pbd::Fluid fluid;
 };
 
+namespace pbd {
+int write_frame_arrays(const char* path, int format, uint32_t frame, uint32_t n, uint32_t width, const float* pos4,
+                       const float* col4, std::string* err);
+}
+
 namespace {
 thread_local std::string g_create_error;
 
@@ -151,6 +156,31 @@ int pbd_compute(pbd_ctx* ctx) {
 int pbd_step_graph(pbd_ctx* ctx, uint32_t frames) {
   if (!ctx) return PBD_ERR_INVALID;
   PBD_TRY(ctx, { return guard_status(ctx, ctx->fluid.step(frames, true)); })
+}
+
+int pbd_write_frame(pbd_ctx* ctx, const char* path, int format, uint32_t frame) {
+  if (!ctx) return PBD_ERR_INVALID;
+  PBD_TRY(ctx, {
+    pbd::Fluid& f = ctx->fluid;
+    const uint32_t n = f.num_particles();
+    std::vector<float> pos((size_t)n * 4), col((size_t)n * 4);
+    if (int rc = f.download(PBD_BUF_POSITIONS, pos.data(), pos.size() * 4)) return rc;
+    if (int rc = f.download(PBD_BUF_COLORS, col.data(), col.size() * 4)) return rc;
+    std::string err;
+    const int rc = pbd::write_frame_arrays(path, format, frame, n, f.params().texture_width, pos.data(), col.data(), &err);
+    if (rc != PBD_OK) return f.report(rc, err);
+    return PBD_OK;
+  })
+}
+
+int pbd_write_frame_arrays(const char* path, int format, uint32_t frame, uint32_t n, uint32_t texture_width,
+                           const float* positions4, const float* colors4) {
+  PBD_TRY(nullptr, {
+    std::string err;
+    const int rc = pbd::write_frame_arrays(path, format, frame, n, texture_width, positions4, colors4, &err);
+    if (rc != PBD_OK) g_create_error = err;
+    return rc;
+  })
 }
 
 int pbd_adj_forward(pbd_ctx* ctx, uint32_t frames, uint32_t k_ckpt) {

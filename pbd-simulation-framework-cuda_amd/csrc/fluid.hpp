@@ -75,6 +75,8 @@ class Fluid : public Cuda_Computable {
   int adj_ids(uint32_t* dst, size_t count);
   int adj_backward(const float* gx, const float* gv, float* gv0, float* grho0, float* geps, float* gx0);
 
+  // Record an error of a call made on this context (pbd_last_error).
+  int report(int code, const std::string& msg) { return fail(code, msg); }
   int status() const { return status_; }
   const std::string& error() const { return error_; }
   void clear_status() {

@@ -169,6 +169,9 @@ _SIGS = {
     "pbd_reset_timings": (C.c_int, [C.c_void_p]),
     "pbd_get_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint32, C.POINTER(C.c_uint32)]),
     "pbd_reset_counters": (C.c_int, [C.c_void_p]),
+    "pbd_write_frame": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_uint32]),
+    "pbd_write_frame_arrays": (C.c_int, [C.c_char_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                         C.c_void_p]),
     "pbd_adj_forward": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "pbd_adj_ids": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "pbd_adj_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float),
@@ -265,6 +268,19 @@ def generate_box(p: PbdParams):
     check(lib().pbd_generate_box(C.byref(p), pos.ctypes.data, vel.ctypes.data, col.ctypes.data, n, C.byref(got)),
           "pbd_generate_box")
     return pos, vel, col
+
+
+OUT_FORMATS = {"raw": 0, "ply": 1, "plyb": 2}  # PBD_OUT_RAW, PBD_OUT_PLY_ASCII, PBD_OUT_PLY_BINARY
+
+
+def write_frame_arrays(path: str, pos4, col4=None, frame: int = 0, texture_width: int = 0, fmt: str = "raw"):
+    """Write one frame from host arrays (pbd_write_frame_arrays)."""
+    pos4 = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
+    col = None if col4 is None else np.ascontiguousarray(col4, np.float32).reshape(-1, 4)
+    rc = lib().pbd_write_frame_arrays(path.encode(), OUT_FORMATS[fmt], frame, pos4.shape[0], texture_width,
+                                      pos4.ctypes.data, None if col is None else col.ctypes.data)
+    if rc != PBD_OK:
+        raise PbdError(rc, "pbd_write_frame_arrays: " + (lib().pbd_last_error(None) or b"").decode())
 
 
 def lattice_rest_density(p: PbdParams) -> float:
@@ -406,6 +422,10 @@ class Context:
 
     def reset_counters(self) -> None:
         self._check(lib().pbd_reset_counters(self._ptr), "pbd_reset_counters")
+
+    def write_frame(self, path: str, fmt: str = "raw", frame: int = 0) -> None:
+        """The current positions4 / colors4 as a frame file (pbd_write_frame)."""
+        self._check(lib().pbd_write_frame(self._ptr, path.encode(), OUT_FORMATS[fmt], frame), "pbd_write_frame")
 
     # ---- reverse mode
     def adj_forward(self, frames: int, k_ckpt: int) -> None:
