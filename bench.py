@@ -229,6 +229,29 @@ def adjoint_leg(F, frames, k_ckpt, seed=5, stabilization=None):
     }
 
 
+def large_scene(F, steps, warmup, config="dam_break_16m"):
+    """The plain step at 16.7M particles (256^3 lattice), whose ~6 GB working set
+    exceeds the 256 MB Infinity Cache that holds most of the 1M scene."""
+    frame = F.load_params(F.config_path(config))
+    p = frame.copy()
+    p.delta_t = frame.delta_t / frame.substeps
+    p.substeps = 1
+    ctx = F.Context(p)
+    n = ctx.n
+    ctx.step(warmup, graph=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(steps, graph=True)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    bad = ctx.check_finite()
+    ctx.close()
+    return {"workload": f"{config}: {n}-particle dam break (256^3 lattice, spacing 0.05, 25.6x12.8x12.8 box), "
+                        f"steps {warmup + 1}-{warmup + steps}",
+            "particles": n, "steps": steps, "particle_steps_per_s": n * steps / el,
+            "ms_per_step": 1e3 * el / steps, "nonfinite_particles": bad}
+
+
 def copy_calibration():
     """Three copies of 256 MiB between device buffers (torch's copy kernel)."""
     import torch
@@ -279,6 +302,8 @@ def main():
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--large-steps", type=int, default=10)
     ap.add_argument("--adjoint-frames", type=int, default=60)
     ap.add_argument("--adjoint-checkpoint", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -349,6 +374,7 @@ def main():
         v["algorithmic_GBps"] = (b / (v["us_per_launch"] * 1e-6) / 1e9) if v["us_per_launch"] else 0.0
     traffic = measured_traffic(dom)
     batched = None if args.no_batched else batched_vs_plain(F, args.scenes, args.batched_steps, 3)
+    large = None if args.no_large else large_scene(F, args.large_steps, 3)
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_leg(F, args.adjoint_frames, args.adjoint_checkpoint)
@@ -395,6 +421,7 @@ def main():
         "nonfinite_particles": bad,
         "neighbour_lists": counters,
         "batched": batched,
+        "large_scene": large,
         "adjoint": adjoint,
     }
     if not args.no_cpu:
