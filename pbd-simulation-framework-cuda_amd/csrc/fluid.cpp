@@ -442,9 +442,11 @@ int Fluid::enqueue_substep() {
   }
   // 9. computeOmega (Fluid.cu:86-88)
   const bool omega_listed = tiles && (g & G_OMEGA);  // the omega pass leaves the lists of Q[cur]
+  // the tile omega pass also applies XSPH: it walks the same neighbours with their velocities
+  const bool xsph_in_omega = omega_listed && (g & G_VISCOSITY);
   if (g & G_OMEGA) {
     if (tiles) {
-      PBD_LAUNCH(ST_OMEGA, launch_tile_omega(st, bits, tgrid, P, TL, B_, B_.Q[cur], keys));
+      PBD_LAUNCH(ST_OMEGA, launch_tile_omega(st, bits, tgrid, P, TL, B_, B_.Q[cur], keys, xsph_in_omega));
     } else {
       PBD_LAUNCH(ST_OMEGA, launch_omega(st, tpb, P, B_, B_.Q[cur], keys));
     }
@@ -453,7 +455,7 @@ int Fluid::enqueue_substep() {
   // 10-11. computeVorticity + computeViscosity (Fluid.cu:90-96) and the output
   uint32_t ff = 0;
   if (g & G_VORTICITY) ff |= FF_VORT;
-  if (g & G_VISCOSITY) ff |= FF_XSPH;
+  if (g & G_VISCOSITY) ff |= xsph_in_omega ? FF_XSPH_DONE : FF_XSPH;
   if (updated) ff |= FF_UPDATED;
   if (tiles) {
     PBD_LAUNCH(ST_FINAL, launch_tile_final(st, bits, tgrid, P, TL, B_, ff, omega_listed, B_.Q[cur], B_.QX[cur], keys));

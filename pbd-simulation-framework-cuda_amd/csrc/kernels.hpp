@@ -12,7 +12,8 @@ namespace pbd {
 // k_delta flags
 enum : uint32_t { DF_APPLY = 1u, DF_UPDATE = 2u, DF_WRITE_DP = 4u };
 // k_final flags
-enum : uint32_t { FF_VORT = 1u, FF_XSPH = 2u, FF_UPDATED = 4u };
+// FF_XSPH_DONE: the omega pass already wrote the XSPH velocities to the output
+enum : uint32_t { FF_VORT = 1u, FF_XSPH = 2u, FF_UPDATED = 4u, FF_XSPH_DONE = 8u };
 
 // Device buffers of one context (see kernels.hip for the layout).
 struct StepBuffers {
@@ -95,8 +96,9 @@ hipError_t launch_tile_lambda(hipStream_t st, int bits, uint32_t grid, const Dev
 hipError_t launch_tile_delta(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
                              const StepBuffers& B, uint32_t flags, bool collide, bool use_list, const float4* Q,
                              const float4* QX, float4* Qo, float4* QXo, const uint32_t* keys);
+// xsph: also apply XSPH (computeViscosity) and write the velocities to B.V
 hipError_t launch_tile_omega(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
-                             const StepBuffers& B, const float4* Q, const uint32_t* keys);
+                             const StepBuffers& B, const float4* Q, const uint32_t* keys, bool xsph);
 hipError_t launch_tile_final(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
                              const StepBuffers& B, uint32_t flags, bool use_list, const float4* Q, const float4* QX,
                              const uint32_t* keys);

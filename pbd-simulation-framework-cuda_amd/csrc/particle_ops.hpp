@@ -266,7 +266,7 @@ __device__ __forceinline__ void final_finish(const DevParams& P, uint32_t flags,
   const float4 xl = (flags & FF_UPDATED) ? qi : QX[s];
   Xo[s] = make_float4(xl.x + cx, xl.y + cy, xl.z + cz, xl.w);
   Po[s] = make_float4(qi.x + cx, qi.y + cy, qi.z + cz, qi.w);
-  Vo[s] = v;
+  if (!(flags & FF_XSPH_DONE)) Vo[s] = v;
   if (P.gates & G_FIND_NEIGHBOURS) {
     const float q = (float)ncnt[s] / (float)P.cap;
     Co[s] = make_float4(q, q, q, 1.0f);
@@ -340,6 +340,25 @@ __device__ __forceinline__ void omega_particle(const DevParams& P, uint32_t s, c
                      omega_pair<VAR>(P, bx - qj.x, by - qj.y, bz - qj.z, vj.x - vi.x, vj.y - vi.y, vj.z - vi.z, a);
                    });
   OM[s] = omega_finish<VAR>(P, a);
+}
+
+// XSPH alone for particle s, with final_particle's neighbour order and arithmetic.
+template <int VAR>
+__device__ __forceinline__ float4 xsph_particle(const DevParams& P, uint32_t s, const float4* __restrict__ Q,
+                                                const float4* __restrict__ VS, const uint32_t* __restrict__ keys,
+                                                const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce) {
+  const float4 qi = Q[s], vi = VS[s];
+  const CellFrame f = make_frame(P, keys[s]);
+  FinalAcc a;
+  uint32_t nc = 0;
+  visit_neighbours(P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0, (P.gates & G_FIND_NEIGHBOURS) != 0, cs,
+                   ce, nc, [&](uint32_t j, float bx, float by, float bz, bool) {
+                     const float4 qj = Q[j];
+                     final_pair_textbook<VAR>(P, false, true, bx - qj.x, by - qj.y, bz - qj.z, vi, VS[j], 0.0f, 0.0f,
+                                              a);
+                   });
+  const float c = P.c_visc * P.wc * P.vol;
+  return make_float4(vi.x + c * a.sx, vi.y + c * a.sy, vi.z + c * a.sz, vi.w + c * a.sw);
 }
 
 template <int VAR>

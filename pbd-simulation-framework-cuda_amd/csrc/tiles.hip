@@ -563,13 +563,17 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_delta(DevParams P, TileLi
   }
 }
 
-template <int BITS>
+// XSPH: the same neighbour walk also applies XSPH (computeViscosity,
+// Density.h:280-310) and writes the velocities to Vo, in the order and with
+// the arithmetic of the final pass, which then leaves them alone.
+template <int BITS, bool XSPH>
 __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileList T, NeighbourList L,
                                                              const float4* __restrict__ Q,
                                                              const float4* __restrict__ VS,
                                                              const uint32_t* __restrict__ keys,
                                                              const uint32_t* __restrict__ cs,
-                                                             const uint32_t* __restrict__ ce, float4* __restrict__ OM) {
+                                                             const uint32_t* __restrict__ ce, float4* __restrict__ OM,
+                                                             float4* __restrict__ Vo) {
   constexpr int VAR = PBD_KERNEL_TEXTBOOK;
   constexpr int CAP = TileCap<BITS>::value;
   __shared__ TileShared<BITS> sh;
@@ -584,6 +588,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileLi
   if (total > (uint32_t)CAP) {
     for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
       omega_particle<VAR>(P, g, Q, VS, keys, cs, ce, OM);
+      if (XSPH) Vo[g] = xsph_particle<VAR>(P, g, Q, VS, keys, cs, ce);
       L.cnt[g] = kNlInvalid;
     }
     if (threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_BLOCK], (unsigned long long)(g1 - g0));
@@ -597,15 +602,23 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileLi
     tile_locate<BITS>(sh, keys[g], g, ix0, iy0, iz0, h, slot);
     if (tile_count<BITS>(sh, h) > P.cap) {
       omega_particle<VAR>(P, g, Q, VS, keys, cs, ce, OM);
+      if (XSPH) Vo[g] = xsph_particle<VAR>(P, g, Q, VS, keys, cs, ce);
       L.cnt[g] = kNlInvalid;
       atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_CAP], 1ull);
       continue;
     }
-    const float4 qi = s_q[slot], vi = VS[g];
+    const float4 qi = s_q[slot];
+    // velocity components as scalars (a float4 aggregate captured by the pair
+    // lambda is not promoted to registers)
+    const float4 v4 = VS[g];
+    const float vx = v4.x, vy = v4.y, vz = v4.z, vw = v4.w;
     OmegaAcc a;
+    FinalAcc xa;
     const auto pair = [&](uint32_t j) __attribute__((always_inline)) {
       const float4 qj = s_q[j], vj = VS[s_gi[j]];
-      omega_pair<VAR>(P, qi.x - qj.x, qi.y - qj.y, qi.z - qj.z, vj.x - vi.x, vj.y - vi.y, vj.z - vi.z, a);
+      const float rx = qi.x - qj.x, ry = qi.y - qj.y, rz = qi.z - qj.z;
+      omega_pair<VAR>(P, rx, ry, rz, vj.x - vx, vj.y - vy, vj.z - vz, a);
+      if (XSPH) final_pair_textbook<VAR>(P, false, true, rx, ry, rz, make_float4(vx, vy, vz, vw), vj, 0.0f, 0.0f, xa);
     };
     const uint32_t n = nl_build<BITS, CAP>(P, L, g, sh, s_q, h, slot, qi);
     L.cnt[g] = (uint16_t)n;
@@ -616,6 +629,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileLi
       atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_LONG], 1ull);
     }
     OM[g] = omega_finish<VAR>(P, a);
+    if (XSPH) {
+      const float c = P.c_visc * P.wc * P.vol;
+      Vo[g] = make_float4(vx + c * xa.sx, vy + c * xa.sy, vz + c * xa.sz, vw + c * xa.sw);
+    }
   }
 }
 
@@ -630,7 +647,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_final(
   constexpr int CAP = TileCap<BITS>::value;
   __shared__ TileShared<BITS> sh;
   __shared__ float4 s_q[tile_lds<BITS>()];
-  __shared__ uint32_t s_gi[CAP + 1];  // global index of each slot: velocities are gathered from L2
+  // global index of each slot, for XSPH's velocity gathers from L2
+  __shared__ uint32_t s_gi[XSPH ? CAP + 1 : 1];
   const uint32_t rank = tile_rank(T);
   if (rank == 0xFFFFFFFFu) return;
   int ix0, iy0, iz0;
@@ -642,8 +660,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_final(
       final_particle<VAR>(P, flags, g, Q, QX, VS, OM, keys, cs, ce, CS, ncnt, Xo, Po, Vo, Co, Fo);
     return;
   }
-  tile_load_gi<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, s_gi, g0,
-                          [&](uint32_t gi) { return vort ? OM[gi].w : 0.0f; });
+  const auto omega_len = [&](uint32_t gi) { return vort ? OM[gi].w : 0.0f; };
+  if (XSPH) tile_load_gi<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, s_gi, g0, omega_len);
+  else tile_load<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, omega_len);
   __syncthreads();
   for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
     const uint32_t key = keys[g];
@@ -738,13 +757,19 @@ hipError_t launch_tile_delta(hipStream_t st, int bits, uint32_t grid, const DevP
 }
 
 hipError_t launch_tile_omega(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
-                             const StepBuffers& B, const float4* Q, const uint32_t* keys) {
+                             const StepBuffers& B, const float4* Q, const uint32_t* keys, bool xsph) {
   dim3 g(grid), b(kTileThreads);
   const NeighbourList L = nlist(B);
-  if (bits == 5)
-    hipLaunchKernelGGL((k_tile_omega<5>), g, b, 0, st, P, T, L, Q, B.VS, keys, B.cstart, B.cend, B.OM);
-  else
-    hipLaunchKernelGGL((k_tile_omega<6>), g, b, 0, st, P, T, L, Q, B.VS, keys, B.cstart, B.cend, B.OM);
+#define PBD_OMEGA(BITS_, X_) \
+  hipLaunchKernelGGL((k_tile_omega<BITS_, X_>), g, b, 0, st, P, T, L, Q, B.VS, keys, B.cstart, B.cend, B.OM, B.V)
+  if (bits == 5) {
+    if (xsph) PBD_OMEGA(5, true);
+    else PBD_OMEGA(5, false);
+  } else {
+    if (xsph) PBD_OMEGA(6, true);
+    else PBD_OMEGA(6, false);
+  }
+#undef PBD_OMEGA
   return hipGetLastError();
 }
 
