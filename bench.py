@@ -48,7 +48,7 @@ def stage_bytes(stage, n, p, table_size, passes):
         "final": 152,
     }
     if stage == "cell_reset":
-        return table_size * 4  # one of the two fill launches
+        return table_size * 8 + ((table_size + 31) // 32) * 4  # cell starts and ends, block flags
     return per.get(stage, 0) * n
 
 

@@ -344,9 +344,8 @@ int Fluid::enqueue_substep() {
   // 1-2. applyForces + initializeCellIds (Fluid.cu:21-25)
   PBD_LAUNCH(ST_PREDICT, launch_predict_keys(st, tpb, P, B_));
   // resetCellInfo (Fluid.cu:31): empty = {UINT_MAX, N}
-  PBD_LAUNCH(ST_CELL_RESET, launch_fill_u32(st, B_.cstart, 0xFFFFFFFFu, B_.table_size));
-  PBD_LAUNCH(ST_CELL_RESET, launch_fill_u32(st, B_.cend, P.n, B_.table_size));
-  if (tiles) PBD_LAUNCH(ST_CELL_RESET, launch_fill_u32(st, tile_flag_, 0u, ntiles));
+  PBD_LAUNCH(ST_CELL_RESET, launch_cell_reset(st, B_.cstart, B_.cend, B_.table_size, P.n, tiles ? tile_flag_ : nullptr,
+                                              tiles ? ntiles : 0u));
   // 3. sortIds (Fluid.cu:27)
   PBD_LAUNCH(ST_SORT, launch_sort(st, sort_, P.n, B_.keys_in, &keys_sorted_, &ids_sorted_));
   // 4-5. reorderStorage + computeCellInfo (Fluid.cu:29,33)

@@ -184,6 +184,39 @@ __global__ void k_check_finite(uint32_t n, const float4* __restrict__ X, const f
 // Host launchers
 static inline uint32_t blocks_for(uint32_t n, uint32_t tpb) { return (n + tpb - 1) / tpb; }
 
+// resetCellInfo (SortReorder.h:140-150) for the whole table in one launch:
+// cstart = UINT_MAX, cend = n, and the occupied-block flags cleared, with 16-byte
+// stores (the arrays come from hipMalloc, 256-byte aligned).
+__global__ void k_cell_reset(uint32_t* __restrict__ cstart, uint32_t* __restrict__ cend, uint32_t ts, uint32_t n,
+                             uint32_t* __restrict__ flags, uint32_t nflags) {
+  const uint32_t q1 = ts / 4u, q2 = nflags / 4u, total = 2u * q1 + q2;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint4 emp = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu), cnt = make_uint4(n, n, n, n);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    if (i < q1) reinterpret_cast<uint4*>(cstart)[i] = emp;
+    else if (i < 2u * q1) reinterpret_cast<uint4*>(cend)[i - q1] = cnt;
+    else reinterpret_cast<uint4*>(flags)[i - 2u * q1] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 4u) {  // tails beyond the last full 16 bytes
+    const uint32_t t = threadIdx.x;
+    if (4u * q1 + t < ts) {
+      cstart[4u * q1 + t] = 0xFFFFFFFFu;
+      cend[4u * q1 + t] = n;
+    }
+    if (flags && 4u * q2 + t < nflags) flags[4u * q2 + t] = 0u;
+  }
+}
+
+hipError_t launch_cell_reset(hipStream_t st, uint32_t* cstart, uint32_t* cend, uint32_t ts, uint32_t n,
+                             uint32_t* flags, uint32_t nflags) {
+  if (!flags) nflags = 0;
+  const uint32_t total = 2u * (ts / 4u) + nflags / 4u + 1u;
+  uint32_t b = blocks_for(total, 256);
+  if (b > 4096) b = 4096;
+  hipLaunchKernelGGL(k_cell_reset, dim3(b), dim3(256), 0, st, cstart, cend, ts, n, flags, nflags);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n) {
   if (n == 0) return hipSuccess;
   uint32_t b = blocks_for(n, 256);

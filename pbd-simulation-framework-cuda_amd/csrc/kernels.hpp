@@ -64,6 +64,9 @@ hipError_t launch_tile_compact(hipStream_t st, uint32_t nflags, const uint32_t* 
                                uint32_t* list, uint32_t* count);
 
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n);
+// cstart[0, ts) = UINT_MAX, cend[0, ts) = n, flags[0, nflags) = 0 (flags may be null)
+hipError_t launch_cell_reset(hipStream_t st, uint32_t* cstart, uint32_t* cend, uint32_t ts, uint32_t n,
+                             uint32_t* flags, uint32_t nflags);
 hipError_t launch_predict_keys(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B);
 // tile_flag[b] = 1 for every occupied Morton block b of 2^tile_bits cells (tile_bits != 0)
 hipError_t launch_reorder(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,
