@@ -244,12 +244,30 @@ def large_scene(F, steps, warmup, config="dam_break_16m"):
     ctx.step(steps, graph=True)
     ctx.sync()
     el = time.perf_counter() - t0
+    # per-stage durations of 3 more steps, HIP events per launch
+    ctx.set_timing(True)
+    ctx.reset_timings()
+    ctx.step(3)
+    ctx.sync()
+    timings = ctx.timings()
+    ctx.set_timing(False)
     bad = ctx.check_finite()
     ctx.close()
+    table = int(F.capi.lib().pbd_morton_encode(p.grid_dims[0] - 1, p.grid_dims[1] - 1, p.grid_dims[2] - 1)) + 1
+    passes = max(1, (table - 1).bit_length() + 7 >> 3)
+    stages = {}
+    for k, (ms, launches) in timings.items():
+        if not launches:
+            continue
+        us = 1e3 * ms / launches
+        b = stage_bytes(k, n, p, table, passes)
+        stages[k] = {"us_per_launch": us, "launches": launches,
+                     "algorithmic_GBps": (b / (us * 1e-6) / 1e9) if us else 0.0,
+                     "hbm_frac": (b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS) if us else 0.0}
     return {"workload": f"{config}: {n}-particle dam break (256^3 lattice, spacing 0.05, 25.6x12.8x12.8 box), "
                         f"steps {warmup + 1}-{warmup + steps}",
             "particles": n, "steps": steps, "particle_steps_per_s": n * steps / el,
-            "ms_per_step": 1e3 * el / steps, "nonfinite_particles": bad}
+            "ms_per_step": 1e3 * el / steps, "stages": stages, "nonfinite_particles": bad}
 
 
 def copy_calibration():

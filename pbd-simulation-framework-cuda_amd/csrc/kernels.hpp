@@ -122,8 +122,11 @@ struct SortPlan {
   uint32_t* vals_out;
 };
 
+#ifndef PBD_SORT_ITEMS
+#define PBD_SORT_ITEMS 16
+#endif
 constexpr uint32_t kSortThreads = 256;
-constexpr uint32_t kSortItems = 8;
+constexpr uint32_t kSortItems = PBD_SORT_ITEMS;  // keys per thread of a onesweep tile
 constexpr uint32_t kSortTile = kSortThreads * kSortItems;
 
 // Number of 8-bit passes covering keys < key_bound.

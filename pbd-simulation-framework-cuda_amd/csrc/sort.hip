@@ -4,7 +4,8 @@
 // (kernel/SortReorder.h:39-47; 7 digit passes x {upsweep, scan, downsweep}).
 // Only the significant bits of the Morton cell id are sorted (ceil(bits/8)
 // 8-bit passes), and each pass is one "onesweep" kernel:
-//   - a tile of 2048 keys per 256-thread workgroup, tile index from an atomic
+//   - a tile of 4096 keys per 256-thread workgroup (16 per thread: digit runs of
+//     ~16 keys per tile keep the scattered writes in 64-byte pieces), tile index from an atomic
 //     ticket so a tile only ever waits on tiles that already started;
 //   - stable in-tile ranking by wave64 multisplit (8 ballots per key give the
 //     lanes with the same digit; rank = popcount below the lane);
