@@ -128,6 +128,14 @@ typedef struct pbd_params {
   /* Application.Cuda.tileBits (new): neighbour passes on LDS tiles of 2^tileBits
    * Morton-aligned cells (5 = 2x4x4, 6 = 4x4x4) or per particle (0) */
   uint32_t tile_bits;
+  /* Unified phase (new, beyond the reference: it has no friction and no solid
+   * phase).  Particles of phase PBD_PHASE_SOLID are granular: no density
+   * constraint, vorticity or XSPH; contacts with them carry position-based
+   * friction (static mu_s, kinetic mu_k; Macklin et al. 2014, eq. 24-25). */
+  float friction_static;               /* Collision.frictionStatic  (mu_s, 0 = no friction) */
+  float friction_kinetic;              /* Collision.frictionKinetic (mu_k) */
+  float box_solid_fraction;            /* Particles.Box.solidFraction: share of the box spawned solid */
+  float box_solid_radius;              /* Particles.Box.solidRadius: radius of the seeded solid spheres */
 } pbd_params;
 
 /* Device pointers of one context (valid until the next pbd_step / add / clear). */
@@ -158,7 +166,8 @@ enum {
   PBD_BUF_CONTACT_COUNTERS = 12,   /* uint32, d_contactCounters */
   PBD_BUF_OMEGAS = 13,          /* float4 (w unused), d_omegas */
   PBD_BUF_DELTA_POSITIONS = 14, /* float4, d_deltaPositions of the last iteration */
-  PBD_BUF_COUNT = 15
+  PBD_BUF_PHASES = 15,          /* uint32, PBD_PHASE_* per particle (new; uploadable) */
+  PBD_BUF_COUNT = 16
 };
 
 /* ---- parameters / config.txt (host only; no GPU needed) ---------------- */
@@ -187,6 +196,11 @@ int pbd_params_write(const pbd_params* p, char* buf, size_t buf_len, size_t* nee
  * point (i,j,k) at box_origin + spacing*(i,j,k) + jitter, i over width (x),
  * j over height (y), k over depth (z), index (i*height + j)*depth + k;
  * v = 0; colour from the seeded RNG with w = 1.  Any output pointer may be NULL. */
+/* Phases of the particles pbd_generate_box spawns, in the same order: the
+ * lattice points inside seeded spheres of radius Box.solidRadius are
+ * PBD_PHASE_SOLID until Box.solidFraction of them are (spheres may overlap). */
+enum { PBD_PHASE_FLUID = 0, PBD_PHASE_SOLID = 1 };
+int pbd_generate_phases(const pbd_params* p, uint32_t* phases, uint32_t capacity, uint32_t* n_out);
 int pbd_generate_box(const pbd_params* p, float* pos4, float* vel4, float* col4,
                      uint32_t capacity, uint32_t* n_out);
 /* Rest density of the spawn lattice under the selected kernel variant. */
@@ -234,7 +248,8 @@ int pbd_sync(pbd_ctx* ctx);
 int pbd_num_particles(pbd_ctx* ctx, uint32_t* n_out);
 int pbd_get_buffers(pbd_ctx* ctx, pbd_buffers* out);
 /* Synchronous host copies of one buffer (PBD_BUF_*); bytes must not exceed
- * the buffer size.  Upload of PBD_BUF_POSITIONS..PBD_BUF_EXTERNAL_FORCES only. */
+ * the buffer size.  Upload of PBD_BUF_POSITIONS..PBD_BUF_EXTERNAL_FORCES and
+ * PBD_BUF_PHASES only. */
 int pbd_download(pbd_ctx* ctx, int which, void* dst, size_t bytes);
 int pbd_upload(pbd_ctx* ctx, int which, const void* src, size_t bytes);
 /* Size in bytes of buffer `which` for the current particle count. */

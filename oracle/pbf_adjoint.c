@@ -290,6 +290,7 @@ static int step64(const ctx64* c, double* X, double* V, double* F, uint32_t* ori
     rc = PBD_ERR_NOMEM;
     goto out;
   }
+  memset(keys0, 0, (size_t)n * 4);
   if (t) {
     t->xin = (double*)malloc((size_t)n * 24);
     t->vin = (double*)malloc((size_t)n * 24);

@@ -219,6 +219,8 @@ int orc_derive(const pbd_params* p, orc_derived* d) {
   d->g_omega = p->active_compute_omegas != 0;
   d->g_vorticity = p->active_compute_vorticity != 0;
   d->g_viscosity = p->active_compute_viscosity != 0;
+  d->mu_s = p->friction_static;
+  d->mu_k = p->friction_kinetic;
   if (p->rest_density_from_lattice)
     d->rho0 = orc_lattice_rest_density(d, p->box_spacing);
   else
@@ -378,7 +380,12 @@ typedef struct orc_work {
   const uint32_t* keys;
   const uint32_t* starts;
   const uint32_t* ends;
+  const uint32_t* ph; /* phases in sorted order, NULL = all fluid */
 } orc_work;
+
+/* Unified phase (beyond the reference): a solid particle has no density
+ * constraint, vorticity or XSPH, and the fluid sums run over fluid neighbours. */
+static inline int solid_at(const orc_work* w, uint32_t i) { return w->ph && w->ph[i] == PBD_PHASE_SOLID; }
 
 static inline void sub3(const double* a, const double* b, double* r) {
   r[0] = a[0] - b[0];
@@ -419,6 +426,7 @@ static double pass_lambda(const orc_work* w, const double* P, double* lambda, do
       double B = 0.0, C[3] = {0, 0, 0};
       for (uint32_t k = 0; k < cnt; ++k) {
         uint32_t j = nb[k];
+        if (solid_at(w, j)) continue;
         double r[3], g[3];
         sub3(&P[4 * i], &P[4 * j], r);
         double rl = len3(r);
@@ -442,7 +450,7 @@ static double pass_lambda(const orc_work* w, const double* P, double* lambda, do
         double s = (dot3(C, C) + B) / (d->rho0 * d->rho0);
         lam = -cc / (s + d->eps);
       }
-      lambda[i] = lam;
+      lambda[i] = solid_at(w, i) ? 0.0 : lam;
       if (density) density[i] = A;
     }
     free(nb);
@@ -464,6 +472,7 @@ static void pass_delta(const orc_work* w, const double* P, const double* lambda,
       const double li = lambda[i];
       for (uint32_t k = 0; k < cnt; ++k) {
         uint32_t j = nb[k];
+        if (solid_at(w, i) || solid_at(w, j)) continue;
         double r[3], g[3];
         sub3(&P[4 * i], &P[4 * j], r);
         double W = orc_w(d, len3(r));
@@ -486,7 +495,7 @@ static void pass_delta(const orc_work* w, const double* P, const double* lambda,
 
 /* solveCollisions (Collision.h:171-242) as a gather over the contact list
  * using the pre-collision predicted positions; pushes p*_i and x_i. */
-static double pass_collide(const orc_work* w, double* P, double* X, double* tmp) {
+static double pass_collide(const orc_work* w, double* P, double* X, double* tmp, double* fric) {
   const orc_derived* d = w->d;
   double minm = DBL_MAX;
 #pragma omp parallel reduction(min : minm)
@@ -496,7 +505,7 @@ static double pass_collide(const orc_work* w, double* P, double* X, double* tmp)
     for (int64_t ii = 0; ii < (int64_t)w->n; ++ii) {
       uint32_t i = (uint32_t)ii, nc;
       orc_neighbours(d, w->n, w->keys, w->starts, w->ends, i, nb, &nc);
-      double push[3] = {0, 0, 0};
+      double push[3] = {0, 0, 0}, fr[3] = {0, 0, 0};
       for (uint32_t k = 0; k < nc; ++k) {
         uint32_t j = nb[k];
         double dir[3];
@@ -511,17 +520,35 @@ static double pass_collide(const orc_work* w, double* P, double* X, double* tmp)
           push[0] += s * dir[0];
           push[1] += s * dir[1];
           push[2] += s * dir[2];
+          /* friction with a solid particle (Macklin et al. 2014, eq. 24-25): the
+           * tangential part t of the relative displacement in the step is removed
+           * when |t| < mu_s d (static), else shortened by mu_k d (kinetic); equal
+           * masses take half each.  Applied to p* only. */
+          if (d->mu_s > 0.0 && (solid_at(w, i) || solid_at(w, j))) {
+            double nrm[3] = {-dir[0] / len, -dir[1] / len, -dir[2] / len}, u[3], t[3];
+            for (int c = 0; c < 3; ++c)
+              u[c] = (P[4 * i + c] - X[4 * i + c]) - (P[4 * j + c] - X[4 * j + c]);
+            const double un = dot3(u, nrm);
+            for (int c = 0; c < 3; ++c) t[c] = u[c] - un * nrm[c];
+            const double tl = len3(t);
+            double f = 1.0;
+            if (!(tl < d->mu_s * ov)) f = fmin(d->mu_k * ov / tl, 1.0);
+            for (int c = 0; c < 3; ++c) fr[c] -= 0.5 * f * t[c];
+          }
         }
       }
       tmp[3 * i + 0] = push[0];
       tmp[3 * i + 1] = push[1];
       tmp[3 * i + 2] = push[2];
+      fric[3 * i + 0] = fr[0];
+      fric[3 * i + 1] = fr[1];
+      fric[3 * i + 2] = fr[2];
     }
     free(nb);
   }
   for (uint32_t i = 0; i < w->n; ++i)
     for (int c = 0; c < 3; ++c) {
-      P[4 * i + c] += tmp[3 * i + c];
+      P[4 * i + c] += tmp[3 * i + c] + fric[3 * i + c];
       X[4 * i + c] += tmp[3 * i + c];
     }
   return minm;
@@ -599,6 +626,7 @@ static void pass_omega(const orc_work* w, const double* P, const double* V, doub
       double om[3] = {0, 0, 0};
       for (uint32_t k = 0; k < cnt; ++k) {
         uint32_t j = nb[k];
+        if (solid_at(w, i) || solid_at(w, j)) continue;
         double r[3], g[3], vij[3], cr[3];
         sub3(&P[4 * i], &P[4 * j], r);
         sub3(&V[4 * j], &V[4 * i], vij);
@@ -642,6 +670,7 @@ static void pass_vorticity(const orc_work* w, const double* P, const double* Om,
         double gs[3] = {0, 0, 0};
         for (uint32_t k = 0; k < cnt; ++k) {
           uint32_t j = nb[k];
+          if (solid_at(w, i) || solid_at(w, j)) continue;
           double dom[3];
           sub3(&Om[3 * j], oi, dom);
           double ol = len3(dom);
@@ -658,6 +687,7 @@ static void pass_vorticity(const orc_work* w, const double* P, const double* Om,
         double wi = len3(oi);
         for (uint32_t k = 0; k < cnt; ++k) {
           uint32_t j = nb[k];
+          if (solid_at(w, i) || solid_at(w, j)) continue;
           double r[3], g[3];
           sub3(&P[4 * i], &P[4 * j], r);
           orc_grad(d, r, g);
@@ -700,6 +730,7 @@ static void pass_xsph(const orc_work* w, const double* P, const double* V, doubl
       double s[4] = {0, 0, 0, 0};
       for (uint32_t k = 0; k < cnt; ++k) {
         uint32_t j = nb[k];
+        if (solid_at(w, i) || solid_at(w, j)) continue;
         double r[3];
         sub3(&P[4 * i], &P[4 * j], r);
         double W = orc_w(d, len3(r));
@@ -753,6 +784,8 @@ int orc_step(const pbd_params* prm, orc_io* io) {
   double* F = (double*)malloc((size_t)n * 32);
   double* Om = (double*)calloc((size_t)n * 3, sizeof(double));
   double* tmp = (double*)malloc((size_t)n * 24);
+  double* fric = (double*)calloc((size_t)n * 3, sizeof(double));
+  uint32_t* phs = io->phase ? (uint32_t*)malloc((size_t)n * 4) : NULL;
   float* c4s = (float*)malloc((size_t)n * 16);
   uint32_t* cnt = (uint32_t*)malloc((size_t)n * 4);
   uint32_t* ccnt = (uint32_t*)malloc((size_t)n * 4);
@@ -761,7 +794,7 @@ int orc_step(const pbd_params* prm, orc_io* io) {
   uint8_t* solved = NULL;
   uint64_t* minc = NULL;
   if (!keys0 || !keys || !ids || !starts || !ends || !X || !P || !V || !Vn || !F || !Om || !tmp ||
-      !c4s || !cnt || !ccnt || !dens) {
+      !c4s || !cnt || !ccnt || !dens || !fric || (io->phase && !phs)) {
     rc = PBD_ERR_NOMEM;
     goto out;
   }
@@ -778,6 +811,7 @@ int orc_step(const pbd_params* prm, orc_io* io) {
       V[4 * s + c] = io->v4[4 * j + c];
       c4s[4 * s + c] = io->c4[4 * j + c];
     }
+    if (phs) phs[s] = io->phase[j];
   }
   /* externalForces, lambdas and deltaPositions are indexed by slot and not reordered */
   for (uint32_t s = 0; s < n; ++s)
@@ -785,7 +819,7 @@ int orc_step(const pbd_params* prm, orc_io* io) {
 
   /* 5. resetCellInfo + computeCellInfo (Fluid.cu:31-33) */
   orc_cell_ranges(n, keys, D.table_size, starts, ends);
-  orc_work W = {&D, n, keys, starts, ends};
+  orc_work W = {&D, n, keys, starts, ends, phs};
 
   /* 6. findContacts / findNeighbours (Fluid.cu:41-47): counts and colours */
   count_pass(&W, cnt, ccnt);
@@ -808,7 +842,7 @@ int orc_step(const pbd_params* prm, orc_io* io) {
     if (D.g_delta) pass_delta(&W, P, io->lambda, io->dp4);
     for (uint32_t sj = 0; sj < D.stab_iters; ++sj) {
       if (D.collision_type == 0) {
-        double m = pass_collide(&W, P, X, tmp);
+        double m = pass_collide(&W, P, X, tmp, fric);
         if (m < io->min_cutoff_margin) io->min_cutoff_margin = m;
       } else {
         if (!nbl) { /* the contact table of the step (cells are fixed within it) */
@@ -872,6 +906,7 @@ int orc_step(const pbd_params* prm, orc_io* io) {
       io->c4[4 * s + c] = c4s[4 * s + c];
     }
   }
+  if (phs) memcpy(io->phase, phs, (size_t)n * 4);
   if (io->out_x4) memcpy(io->out_x4, X, (size_t)n * 32);
   if (io->out_p4) memcpy(io->out_p4, P, (size_t)n * 32);
   if (io->out_v4) memcpy(io->out_v4, V, (size_t)n * 32);
@@ -898,6 +933,8 @@ out:
   free(F);
   free(Om);
   free(tmp);
+  free(fric);
+  free(phs);
   free(c4s);
   free(cnt);
   free(ccnt);

@@ -62,6 +62,7 @@ typedef struct orc_derived {
   uint32_t solver_iters, stab_iters, max_batches;
   int g_apply_forces, g_find_contacts, g_find_neighbours, g_lambda, g_delta,
       g_apply_delta, g_update, g_omega, g_vorticity, g_viscosity;
+  double mu_s, mu_k;           /* Collision.frictionStatic / frictionKinetic */
 } orc_derived;
 
 /* In/out arrays of one step.  State arrays (float) are read and updated in
@@ -93,6 +94,8 @@ typedef struct orc_io {
   uint32_t* out_contact_count; /* n */
   /* statistics of pair distances seen by support tests (min |r - cutoff|) */
   double min_cutoff_margin;
+  /* phases (PBD_PHASE_*, n; NULL = all fluid), updated in place (sorted order after the step) */
+  uint32_t* phase;
 } orc_io;
 
 /* ---- building blocks ---- */

@@ -39,6 +39,7 @@ class OrcDerived(C.Structure):
         ("g_apply_forces", C.c_int), ("g_find_contacts", C.c_int), ("g_find_neighbours", C.c_int),
         ("g_lambda", C.c_int), ("g_delta", C.c_int), ("g_apply_delta", C.c_int), ("g_update", C.c_int),
         ("g_omega", C.c_int), ("g_vorticity", C.c_int), ("g_viscosity", C.c_int),
+        ("mu_s", C.c_double), ("mu_k", C.c_double),
     ]
 
 
@@ -52,6 +53,7 @@ class OrcIo(C.Structure):
         ("out_keys_unsorted", _up), ("out_keys", _up), ("out_ids", _up), ("out_starts", _up), ("out_ends", _up),
         ("out_nbr_count", _up), ("out_contact_count", _up),
         ("min_cutoff_margin", C.c_double),
+        ("phase", _up),
     ]
 
 
@@ -160,7 +162,7 @@ def neighbours(d: OrcDerived, keys_sorted, starts, ends, i: int):
 class State:
     """FP32 particle state of the oracle (the reference's surfaces + persistent arrays)."""
 
-    def __init__(self, x4, v4=None, c4=None, f4=None, p4=None):
+    def __init__(self, x4, v4=None, c4=None, f4=None, p4=None, phase=None):
         x4 = np.ascontiguousarray(x4, np.float32).reshape(-1, 4)
         n = x4.shape[0]
         self.n = n
@@ -174,6 +176,7 @@ class State:
                    else np.ascontiguousarray(f4, np.float32).reshape(-1, 4).copy())
         self.lam = np.zeros(n, np.float64)
         self.dp4 = np.zeros((n, 4), np.float64)
+        self.phase = None if phase is None else np.ascontiguousarray(phase, np.uint32).copy()
 
 
 def step(p: PbdParams, st: State, want_outputs: bool = True) -> dict:
@@ -185,6 +188,7 @@ def step(p: PbdParams, st: State, want_outputs: bool = True) -> dict:
     io.x4, io.p4, io.v4 = _ptr(st.x4, _fp), _ptr(st.p4, _fp), _ptr(st.v4, _fp)
     io.c4, io.f4 = _ptr(st.c4, _fp), _ptr(st.f4, _fp)
     io.lam, io.dp4 = _ptr(st.lam, _dp), _ptr(st.dp4, _dp)
+    io.phase = _ptr(st.phase, _up)
     out = {}
     if want_outputs:
         out = dict(

@@ -66,6 +66,7 @@ int Fluid::adj_check() {
       !p.active_compute_omegas || !p.active_compute_vorticity || !p.active_compute_viscosity)
     return fail(PBD_ERR_UNSUPPORTED, "adjoint: every Active.* stage but findNeighbours must be on");
   if (scene_table_) return fail(PBD_ERR_UNSUPPORTED, "adjoint: not available for batched scenes");
+  if (phased()) return fail(PBD_ERR_UNSUPPORTED, "adjoint: not available with solid particles or friction");
   if (n_ == 0) return fail(PBD_ERR_INVALID, "adjoint: no particles");
   return PBD_OK;
 }
@@ -78,7 +79,7 @@ int Fluid::adj_forward(uint32_t frames, uint32_t k_ckpt) {
   if (!dev_valid_) {
     const char* why = nullptr;
     if (derive_device_params(params_, n_, dev_, &why) != PBD_OK) return fail(PBD_ERR_INVALID, why);
-    apply_scenes();
+    apply_context();
     dev_valid_ = true;
   }
   adj_release();

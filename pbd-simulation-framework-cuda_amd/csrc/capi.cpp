@@ -131,7 +131,13 @@ int pbd_spawn_box(pbd_ctx* ctx) {
     uint32_t n = 0;
     int rc = pbd_generate_box(&p, pos.data(), vel.data(), col.data(), (uint32_t)total, &n);
     if (rc != PBD_OK) return rc;
-    return pbd_add_particles(ctx, n, pos.data(), vel.data(), col.data());
+    const uint32_t first = ctx->fluid.num_particles();
+    rc = pbd_add_particles(ctx, n, pos.data(), vel.data(), col.data());
+    if (rc != PBD_OK || !(p.box_solid_fraction > 0.0f)) return rc;
+    std::vector<uint32_t> ph(total);
+    rc = pbd_generate_phases(&p, ph.data(), (uint32_t)total, &n);
+    if (rc != PBD_OK) return rc;
+    return ctx->fluid.set_phases(first, n, ph.data());
   })
 }
 

@@ -288,6 +288,8 @@ const KeyDesc kKeys[] = {
     {"Application.Simulation.Collision.kernelWidthNeighbours", KT::U32, OFF(kernel_width_neighbours), true, true, 1, 50, false, false},
     {"Application.Simulation.Collision.maxCollisionBatches", KT::U32, OFF(max_collision_batches), true, true, 1, 512, false, false},
     {"Application.Simulation.Collision.stiffness", KT::F32, OFF(stiffness), true, true, 0, 1, false, false},
+    {"Application.Simulation.Collision.frictionStatic", KT::F32, OFF(friction_static), false, true, 0, 100, false, false},
+    {"Application.Simulation.Collision.frictionKinetic", KT::F32, OFF(friction_kinetic), false, true, 0, 100, false, false},
     {"Application.Simulation.Density.kSCorr", KT::F32, OFF(k_scorr), true, true, -100, 100, false, false},
     {"Application.Simulation.Density.kernelWidthSpiky", KT::F32, OFF(kernel_width_spiky), true, true, 0, 1000, true, false},
     {"Application.Simulation.Density.nSCorr", KT::I32, OFF(n_scorr), true, true, -100, 100, false, false},
@@ -319,6 +321,8 @@ const KeyDesc kKeys[] = {
     {"Application.Simulation.Particles.Box.origin", KT::F32X3, OFF(box_origin), false, false, 0, 0, false, false},
     {"Application.Simulation.Particles.Box.jitter", KT::F32, OFF(box_jitter), false, true, 0, 1e6, false, false},
     {"Application.Simulation.Particles.Box.seed", KT::SEED, OFF(box_seed_lo), false, false, 0, 0, false, false},
+    {"Application.Simulation.Particles.Box.solidFraction", KT::F32, OFF(box_solid_fraction), false, true, 0, 1, false, false},
+    {"Application.Simulation.Particles.Box.solidRadius", KT::F32, OFF(box_solid_radius), false, true, 0, 1e6, true, false},
     {"Application.Simulation.Viscosity.cViscosity", KT::F32, OFF(c_viscosity), true, true, 0, 100, false, false},
     {"Application.Simulation.Viscosity.kernelWidthPoly", KT::F32, OFF(kernel_width_poly), true, true, 0, 1000, true, false},
     {"Application.Simulation.Vorticity.eVorticity", KT::F32, OFF(e_vorticity), true, true, -10000, 10000, false, false},
@@ -678,6 +682,11 @@ int derive_device_params(const pbd_params& p, uint32_t n, DevParams& o, const ch
     o.list_r2 = r * r * 1.0001f;
   }
   o.e_vort = p.e_vorticity;
+  o.mu_s = p.friction_static;
+  o.mu_k = p.friction_kinetic;
+  o.phased = 0;  // set by the context when it holds solid particles or friction is on
+  o.phs = nullptr;
+  o.ph = nullptr;
   o.scenes = nullptr;  // one scene; a batched context installs its scene table
   o.n_scenes = 1;
   o.slot_log2 = 10;
@@ -751,6 +760,10 @@ extern "C" int pbd_params_default(pbd_params* o) {
     o->grid_dims[a] = 128;  // maxGrid = 128^3 (Particles.cc:8)
   }
   o->tile_bits = 5;
+  o->friction_static = 0.0f;
+  o->friction_kinetic = 0.0f;
+  o->box_solid_fraction = 0.0f;
+  o->box_solid_radius = 0.1f;
   return PBD_OK;
 }
 
@@ -882,6 +895,39 @@ extern "C" int pbd_generate_box(const pbd_params* p, float* pos4, float* vel4, f
           col4[4 * idx + 3] = 1.0f;
         }
       }
+  return PBD_OK;
+}
+
+extern "C" int pbd_generate_phases(const pbd_params* p, uint32_t* phases, uint32_t capacity, uint32_t* n_out) {
+  if (!p) return PBD_ERR_INVALID;
+  const uint64_t W = p->box_width, H = p->box_height, D = p->box_depth, total = W * H * D;
+  if (n_out) *n_out = (uint32_t)total;
+  if (total > capacity) return PBD_ERR_CAPACITY;
+  if (!phases) return PBD_OK;
+  for (uint64_t i = 0; i < total; ++i) phases[i] = PBD_PHASE_FLUID;
+  const double frac = p->box_solid_fraction;
+  if (!(frac > 0.0) || total == 0) return PBD_OK;
+  const uint64_t target = (uint64_t)std::llround(frac * (double)total);
+  // sphere centres in lattice units, uniform over the box; their own stream of the box seed
+  uint64_t st = (((uint64_t)p->box_seed_hi << 32) | p->box_seed_lo) ^ 0x5DEECE66DA3B9F27ull;
+  const double rl = (double)p->box_solid_radius / (double)p->box_spacing, r2 = rl * rl;
+  uint64_t marked = 0;
+  for (uint32_t tries = 0; marked < target && tries < 10000000u; ++tries) {
+    const double c[3] = {uniform01(st) * (double)W, uniform01(st) * (double)H, uniform01(st) * (double)D};
+    const int64_t lo[3] = {(int64_t)std::floor(c[0] - rl), (int64_t)std::floor(c[1] - rl), (int64_t)std::floor(c[2] - rl)};
+    const int64_t hi[3] = {(int64_t)std::ceil(c[0] + rl), (int64_t)std::ceil(c[1] + rl), (int64_t)std::ceil(c[2] + rl)};
+    for (int64_t i = std::max<int64_t>(lo[0], 0); i <= std::min<int64_t>(hi[0], (int64_t)W - 1); ++i)
+      for (int64_t j = std::max<int64_t>(lo[1], 0); j <= std::min<int64_t>(hi[1], (int64_t)H - 1); ++j)
+        for (int64_t k = std::max<int64_t>(lo[2], 0); k <= std::min<int64_t>(hi[2], (int64_t)D - 1); ++k) {
+          const double dx = (double)i - c[0], dy = (double)j - c[1], dz = (double)k - c[2];
+          if (dx * dx + dy * dy + dz * dz > r2) continue;
+          uint32_t& ph = phases[((uint64_t)i * H + (uint64_t)j) * D + (uint64_t)k];
+          if (ph == PBD_PHASE_FLUID) {
+            ph = PBD_PHASE_SOLID;
+            ++marked;
+          }
+        }
+  }
   return PBD_OK;
 }
 

@@ -106,6 +106,8 @@ int Fluid::init(const pbd_params& p, uint32_t capacity) {
   rc |= dev_alloc(allocations_, &B_.NCNT, c);
   rc |= dev_alloc(allocations_, &B_.CCNT, c);
   rc |= dev_alloc(allocations_, &B_.keys_in, c);
+  rc |= dev_alloc(allocations_, &ph_, c);
+  rc |= dev_alloc(allocations_, &phs_, c);
   sort_.n_max = cap_;
   sort_.tiles_max = (cap_ + kSortTile - 1) / kSortTile;
   rc |= dev_alloc(allocations_, &sort_.hist, 4 * 256);
@@ -130,7 +132,7 @@ int Fluid::init(const pbd_params& p, uint32_t capacity) {
   for (void* q : {(void*)B_.X, (void*)B_.P, (void*)B_.V, (void*)B_.C, (void*)B_.F, (void*)B_.DP, (void*)B_.OM,
                   (void*)B_.VS, (void*)B_.CS})
     if (e == hipSuccess) e = hipMemsetAsync(q, 0, c * 16, stream_);
-  for (void* q : {(void*)B_.LAM, (void*)B_.DENS, (void*)B_.NCNT, (void*)B_.CCNT})
+  for (void* q : {(void*)B_.LAM, (void*)B_.DENS, (void*)B_.NCNT, (void*)B_.CCNT, (void*)ph_, (void*)phs_})
     if (e == hipSuccess) e = hipMemsetAsync(q, 0, c * 4, stream_);
   if (e == hipSuccess) e = hipMemsetAsync(sort_.err, 0, 4, stream_);
   if (e == hipSuccess) e = hipMemsetAsync(d_scratch_, 0, 16, stream_);
@@ -183,6 +185,7 @@ int Fluid::add_particles(uint32_t n, const float* pos4, const float* vel4, const
   if (e == hipSuccess) e = hipMemcpy(B_.V + off, vel4, bytes, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(B_.C + off, col4, bytes, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(B_.F + off, f0.data(), bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(ph_ + off, 0, (size_t)n * 4);  // new particles are fluid
   if (hip_check(e, "add_particles upload")) return status_;
   n_ += n;
   dev_valid_ = false;
@@ -283,7 +286,12 @@ void Fluid::set_scenes(const SceneParams* table, uint32_t n, uint32_t slot_log2)
   invalidate_graph();
 }
 
-void Fluid::apply_scenes() {
+// Context tables the derived parameters point to: the phases (when the context
+// holds solid particles or friction is on) and a batched context's scene table.
+void Fluid::apply_context() {
+  dev_.phased = (has_solids_ || params_.friction_static > 0.0f) ? 1u : 0u;
+  dev_.phs = dev_.phased ? phs_ : nullptr;
+  dev_.ph = dev_.phased ? ph_ : nullptr;
   if (!scene_table_) return;
   dev_.scenes = scene_table_;
   dev_.n_scenes = scene_count_;
@@ -328,7 +336,7 @@ int Fluid::enqueue_substep() {
   const uint32_t g = P.gates;
   // LDS tiles for the 27-cell stencil of the textbook kernels (tiles.hip)
   const int bits = (int)params_.tile_bits;
-  const bool tiles = bits != 0 && P.variant == PBD_KERNEL_TEXTBOOK && P.kw == 1 && (g & G_FIND_CONTACTS);
+  const bool tiles = bits != 0 && P.variant == PBD_KERNEL_TEXTBOOK && P.kw == 1 && (g & G_FIND_CONTACTS) && !P.phased;
   uint32_t tgrid = 0, ntiles = 0;
   if (tiles) {
     ntiles = ((B_.table_size - 1u) >> bits) + 1u;
@@ -472,7 +480,7 @@ void Fluid::compute() {
       fail(PBD_ERR_INVALID, why);
       return;
     }
-    apply_scenes();
+    apply_context();
     dev_valid_ = true;
   }
   enqueue_substep();
@@ -503,7 +511,7 @@ int Fluid::ensure_graph() {
   if (!dev_valid_) {
     const char* why = nullptr;
     if (derive_device_params(params_, n_, dev_, &why) != PBD_OK) return fail(PBD_ERR_INVALID, why);
-    apply_scenes();
+    apply_context();
     dev_valid_ = true;
   }
   if (hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
@@ -567,6 +575,9 @@ static int elem_info(int which, size_t* elem, bool* table) {
       *elem = 4;
       *table = true;
       return PBD_OK;
+    case PBD_BUF_PHASES:
+      *elem = 4;
+      return PBD_OK;
     default:
       return PBD_ERR_INVALID;
   }
@@ -602,6 +613,7 @@ int Fluid::download(int which, void* dst, size_t bytes) {
     case PBD_BUF_CONTACT_COUNTERS: src = B_.CCNT; break;
     case PBD_BUF_OMEGAS: src = B_.OM; break;
     case PBD_BUF_DELTA_POSITIONS: src = B_.DP; break;
+    case PBD_BUF_PHASES: src = ph_; break;
   }
   if (bytes == 0) return PBD_OK;
   return hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "download");
@@ -624,6 +636,7 @@ int Fluid::download_range(int which, uint32_t first, uint32_t count, void* dst) 
     case PBD_BUF_DELTA_POSITIONS: src = B_.DP; esz = 16; break;
     case PBD_BUF_CELL_IDS: src = keys_sorted_; esz = 4; break;
     case PBD_BUF_PARTICLE_IDS: src = ids_sorted_; esz = 4; break;
+    case PBD_BUF_PHASES: src = ph_; esz = 4; break;
     default: return fail(PBD_ERR_INVALID, "download: not a per-particle buffer");
   }
   if ((uint64_t)first + count > n_ || (count && !dst)) return fail(PBD_ERR_INVALID, "download: range exceeds particle count");
@@ -634,7 +647,30 @@ int Fluid::download_range(int which, uint32_t first, uint32_t count, void* dst) 
                    "download");
 }
 
+int Fluid::set_phases(uint32_t first, uint32_t count, const uint32_t* phases) {
+  if ((uint64_t)first + count > n_ || (count && !phases)) return fail(PBD_ERR_INVALID, "phases: range exceeds particle count");
+  bool solid = false;
+  for (uint32_t i = 0; i < count; ++i) {
+    if (phases[i] > PBD_PHASE_SOLID) return fail(PBD_ERR_INVALID, "phases: values must be PBD_PHASE_FLUID or PBD_PHASE_SOLID");
+    solid |= phases[i] == PBD_PHASE_SOLID;
+  }
+  if (int rc = sync()) return rc;
+  if (count && hip_check(hipMemcpy(ph_ + first, phases, (size_t)count * 4, hipMemcpyHostToDevice), "phases upload"))
+    return status_;
+  // the per-particle path with phase masks stays on once a solid particle was seen
+  if (solid && !has_solids_) {
+    has_solids_ = true;
+    dev_valid_ = false;
+    invalidate_graph();
+  }
+  return PBD_OK;
+}
+
 int Fluid::upload(int which, const void* src, size_t bytes) {
+  if (which == PBD_BUF_PHASES) {
+    if (bytes > (size_t)n_ * 4 || (bytes % 4) != 0) return fail(PBD_ERR_INVALID, "upload: size exceeds particle count");
+    return set_phases(0, (uint32_t)(bytes / 4), static_cast<const uint32_t*>(src));
+  }
   void* dst = nullptr;
   switch (which) {
     case PBD_BUF_POSITIONS: dst = B_.X; break;

@@ -54,6 +54,9 @@ class Fluid : public Cuda_Computable {
   // count elements of a per-particle buffer from index first (sorted order)
   int download_range(int which, uint32_t first, uint32_t count, void* dst);
   int upload(int which, const void* src, size_t bytes);
+  // phases of particles [first, first + count) (PBD_PHASE_*)
+  int set_phases(uint32_t first, uint32_t count, const uint32_t* phases);
+  bool phased() const { return has_solids_ || params_.friction_static > 0.0f; }
   int buffer_bytes(int which, size_t* bytes) const;
   int check_finite(uint32_t* bad);
   void get_buffers(pbd_buffers* out) const;
@@ -89,7 +92,7 @@ class Fluid : public Cuda_Computable {
   int hip_check(hipError_t e, const char* what);
   int alloc_table();
   int ensure_batch_buffers();
-  void apply_scenes();
+  void apply_context();
   int enqueue_substep();
   int ensure_graph();
   int adj_check();
@@ -122,6 +125,9 @@ class Fluid : public Cuda_Computable {
   uint32_t batch_words_ = 0;
   const SceneParams* scene_table_ = nullptr;  // batched scenes
   uint32_t scene_count_ = 1, slot_log2_ = 10;
+  uint32_t* ph_ = nullptr;   // public phases (PBD_BUF_PHASES)
+  uint32_t* phs_ = nullptr;  // phases in the step's sorted order
+  bool has_solids_ = false;
 
   // reverse mode: checkpoints, the record of one segment and the work
   // buffers (allocated by adj_forward); rec_ is the step record being written

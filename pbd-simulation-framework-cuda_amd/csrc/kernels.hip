@@ -89,6 +89,7 @@ __global__ void k_reorder(DevParams P, const uint32_t* __restrict__ keys_s, cons
   QX[s] = make_float4(x.x - cx, x.y - cy, x.z - cz, x.w);
   VS[s] = v;
   if (!(P.gates & G_FIND_NEIGHBOURS)) CS[s] = C[j];
+  if (P.phs) P.phs[s] = P.ph[j];
 }
 
 // ---------------------------------------------------------------------------
@@ -168,6 +169,7 @@ __global__ void k_final(DevParams P, uint32_t flags, const float4* __restrict__ 
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.n) return;
   final_particle<VAR>(P, flags, s, Q, QX, VS, OM, keys, cs, ce, CS, ncnt, Xo, Po, Vo, Co, Fo);
+  if (P.phs) P.ph[s] = P.phs[s];  // phases follow the particles into the sorted order
 }
 
 // Finite check of positions and velocities (failure detection).

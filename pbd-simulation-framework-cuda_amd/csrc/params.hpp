@@ -70,6 +70,12 @@ struct DevParams {
   float e_vort;
   float inv_dt;
   float list_r2;           // neighbour-list radius^2: max(h_poly, h_spiky, diameter)^2 (padded)
+  // unified phase: phases are read (PHS) when phased != 0; friction on contacts
+  // with a solid particle when mu_s > 0
+  uint32_t phased;
+  float mu_s, mu_k;
+  uint32_t* phs;  // phases in the step's sorted order (nullptr unless phased)
+  uint32_t* ph;   // public phases (previous order in, sorted order out)
   // batched scenes: table of n_scenes entries (nullptr: one scene, the fields above);
   // the scene of a particle is the Morton index of its slot, cell key >> 3 slot_log2
   const SceneParams* scenes;

@@ -46,7 +46,30 @@ __device__ __forceinline__ float lambda_finish(const DevParams& P, const LambdaA
 struct DeltaAcc {
   float ax = 0.0f, ay = 0.0f, az = 0.0f;  // sum (l_i + l_j + s_corr) g r
   float px = 0.0f, py = 0.0f, pz = 0.0f;  // collision push
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;  // friction (applied to p* only)
 };
+
+// Position-based friction of one contact with a solid particle (Macklin et al.
+// 2014, eq. 24-25; the reference has no friction).  r = p*_i - p*_j (len, 1/len),
+// (dix, diy, diz) and dj = p* - x of i and j: their displacement in the step.
+// The tangential part t of the relative displacement is removed when
+// |t| < mu_s d (static) and shortened by mu_k d otherwise (kinetic), d = diameter
+// - len the penetration; equal masses take half each.
+__device__ __forceinline__ void friction_pair(const DevParams& P, float rx, float ry, float rz, float len, float il,
+                                              float dix, float diy, float diz, float djx, float djy, float djz,
+                                              DeltaAcc& a) {
+  const float nx = rx * il, ny = ry * il, nz = rz * il;
+  const float ux = dix - djx, uy = diy - djy, uz = diz - djz;
+  const float un = ux * nx + uy * ny + uz * nz;
+  const float tx = ux - un * nx, ty = uy - un * ny, tz = uz - un * nz;
+  const float tl = sqrtf(tx * tx + ty * ty + tz * tz);
+  const float d = P.diameter - len;
+  float s = 1.0f;
+  if (!(tl < P.mu_s * d)) s = fminf(P.mu_k * d / tl, 1.0f);  // tl > 0 here: mu_s d > 0
+  a.fx -= 0.5f * s * tx;
+  a.fy -= 0.5f * s * ty;
+  a.fz -= 0.5f * s * tz;
+}
 
 // s_corr exponent: NS > 0 compile-time, NS == 0 the runtime nSCorr.
 template <int NS>
@@ -129,7 +152,7 @@ __device__ __forceinline__ float4 delta_finish(const DevParams& P, uint32_t flag
     dz = d.z;
   }
   const float4 qxi = QX[s];
-  float nx = qi.x + a.px, ny = qi.y + a.py, nz = qi.z + a.pz;
+  float nx = qi.x + a.px + a.fx, ny = qi.y + a.py + a.fy, nz = qi.z + a.pz + a.fz;
   float4 qx = make_float4(qxi.x + a.px, qxi.y + a.py, qxi.z + a.pz, qxi.w);
   if (flags & DF_APPLY) {
     if (VAR == PBD_KERNEL_REFERENCE) {
@@ -276,12 +299,15 @@ __device__ __forceinline__ void final_finish(const DevParams& P, uint32_t flags,
 }
 
 // ---------------------------------------------------------------- per-particle (global) bodies
+// P.phs: phases in sorted order (nullptr: all fluid).  A solid particle has no
+// density constraint (lambda 0) and a fluid one counts only fluid neighbours.
 template <int VAR, bool FIRST>
 __device__ __forceinline__ void lambda_particle(const DevParams& P, uint32_t s, const float4* __restrict__ Q,
                                                 const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
                                                 const uint32_t* __restrict__ ce, float* __restrict__ lam,
                                                 float* __restrict__ dens, uint32_t* __restrict__ ncnt,
                                                 uint32_t* __restrict__ ccnt) {
+  const uint32_t* __restrict__ ph = P.phs;
   const float4 qi = Q[s];
   const CellFrame f = make_frame(P, keys[s]);
   LambdaAcc a;
@@ -289,11 +315,13 @@ __device__ __forceinline__ void lambda_particle(const DevParams& P, uint32_t s, 
   const uint32_t cnt = visit_neighbours(P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0,
                                         (P.gates & G_FIND_NEIGHBOURS) != 0, cs, ce, nc,
                                         [&](uint32_t j, float bx, float by, float bz, bool) {
+                                          if (ph && ph[j]) return;
                                           const float4 qj = Q[j];
                                           lambda_pair<VAR>(P, bx - qj.x, by - qj.y, bz - qj.z, a);
                                         });
   float W;
   lam[s] = lambda_finish<VAR>(P, a, W, scene_eps(P, keys[s]));
+  if (ph && ph[s]) lam[s] = 0.0f;
   if (FIRST) {
     dens[s] = W;
     ncnt[s] = cnt;
@@ -301,6 +329,8 @@ __device__ __forceinline__ void lambda_particle(const DevParams& P, uint32_t s, 
   }
 }
 
+// P.phs: phases (nullptr: all fluid).  Delta p acts between fluid particles only;
+// the contact push acts on every pair and friction on contacts with a solid.
 template <int VAR, bool DELTA, bool COLLIDE>
 __device__ __forceinline__ float4 delta_particle(const DevParams& P, uint32_t flags, uint32_t s,
                                                const float4* __restrict__ Q, const float4* __restrict__ QX,
@@ -308,34 +338,65 @@ __device__ __forceinline__ float4 delta_particle(const DevParams& P, uint32_t fl
                                                const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
                                                float4* __restrict__ Qo, float4* __restrict__ QXo,
                                                float4* __restrict__ DP, float4* __restrict__ VS) {
+  const uint32_t* __restrict__ ph = P.phs;
   const float4 qi = Q[s];
   const CellFrame f = make_frame(P, keys[s]);
   DeltaAcc a;
   if (DELTA || COLLIDE) {
     const float li = DELTA ? lam[s] : 0.0f;
+    const bool si = ph && ph[s];
+    const bool fric = COLLIDE && ph && P.mu_s > 0.0f;
+    float dix = 0.0f, diy = 0.0f, diz = 0.0f;
+    if (fric) {
+      const float4 x = QX[s];
+      dix = qi.x - x.x;
+      diy = qi.y - x.y;
+      diz = qi.z - x.z;
+    }
     uint32_t nc = 0;
     visit_neighbours(P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0,
                      DELTA && (P.gates & G_FIND_NEIGHBOURS) != 0, cs, ce, nc,
                      [&](uint32_t j, float bx, float by, float bz, bool contact) {
                        const float4 qj = Q[j];
-                       delta_pair<VAR, DELTA, COLLIDE>(P, li, DELTA ? lam[j] : 0.0f, bx - qj.x, by - qj.y, bz - qj.z,
-                                                       contact, a);
+                       const float rx = bx - qj.x, ry = by - qj.y, rz = bz - qj.z;
+                       const bool sj = ph && ph[j];
+                       if (si || sj) {
+                         delta_pair<VAR, false, COLLIDE>(P, 0.0f, 0.0f, rx, ry, rz, contact, a);
+                         if (fric && contact) {
+                           const float r2 = rx * rx + ry * ry + rz * rz;
+                           if (r2 > 0.0f) {
+                             const float il = rsq(r2), len = r2 * il;
+                             if (len < P.diameter && len > 0.00001f) {
+                               const float4 xj = QX[j];
+                               friction_pair(P, rx, ry, rz, len, il, dix, diy, diz, qj.x - xj.x, qj.y - xj.y,
+                                             qj.z - xj.z, a);
+                             }
+                           }
+                         }
+                       } else {
+                         delta_pair<VAR, DELTA, COLLIDE>(P, li, DELTA ? lam[j] : 0.0f, rx, ry, rz, contact, a);
+                       }
                      });
   }
   return delta_finish<VAR, DELTA>(P, flags, s, qi, QX, a, f.cx, f.cy, f.cz, keys[s], Qo, QXo, DP, VS);
 }
 
+// P.phs: phases (nullptr: all fluid); omega of a solid particle is 0, a fluid one
+// sums over fluid neighbours.
 template <int VAR>
 __device__ __forceinline__ void omega_particle(const DevParams& P, uint32_t s, const float4* __restrict__ Q,
                                                const float4* __restrict__ VS, const uint32_t* __restrict__ keys,
                                                const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
                                                float4* __restrict__ OM) {
+  const uint32_t* __restrict__ ph = P.phs;
   const float4 qi = Q[s], vi = VS[s];
   const CellFrame f = make_frame(P, keys[s]);
   OmegaAcc a;
   uint32_t nc = 0;
+  const bool si = ph && ph[s];
   visit_neighbours(P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0, (P.gates & G_FIND_NEIGHBOURS) != 0, cs,
                    ce, nc, [&](uint32_t j, float bx, float by, float bz, bool) {
+                     if (si || (ph && ph[j])) return;
                      const float4 qj = Q[j], vj = VS[j];
                      omega_pair<VAR>(P, bx - qj.x, by - qj.y, bz - qj.z, vj.x - vi.x, vj.y - vi.y, vj.z - vi.z, a);
                    });
@@ -370,16 +431,19 @@ __device__ __forceinline__ void final_particle(const DevParams& P, uint32_t flag
                                                const uint32_t* __restrict__ ncnt, float4* __restrict__ Xo,
                                                float4* __restrict__ Po, float4* __restrict__ Vo,
                                                float4* __restrict__ Co, float4* __restrict__ Fo) {
+  const uint32_t* __restrict__ ph = P.phs;
   const float4 qi = Q[s], vi = VS[s];
   const CellFrame f = make_frame(P, keys[s]);
   const bool vort = (flags & FF_VORT) != 0, xsph = (flags & FF_XSPH) != 0;
   const float4 oi = vort ? OM[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const bool si = ph && ph[s];  // solid: no vorticity force, no XSPH
   FinalAcc a;
   if (vort || xsph) {
     uint32_t nc = 0;
     visit_neighbours(P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0,
                      (P.gates & G_FIND_NEIGHBOURS) != 0, cs, ce, nc,
                      [&](uint32_t j, float bx, float by, float bz, bool) {
+                       if (si || (ph && ph[j])) return;
                        const float4 qj = Q[j];
                        const float4 vj = xsph ? VS[j] : vi;
                        const float rx = bx - qj.x, ry = by - qj.y, rz = bz - qj.z;

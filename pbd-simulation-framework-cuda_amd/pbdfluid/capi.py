@@ -43,11 +43,14 @@ BUF = dict(
     contact_counters=12,
     omegas=13,
     delta_positions=14,
+    phases=15,
 )
+PHASE_FLUID, PHASE_SOLID = 0, 1
 _BUF_DTYPE = {
     0: (np.float32, 4), 1: (np.float32, 4), 2: (np.float32, 4), 3: (np.float32, 4), 4: (np.float32, 4),
     5: (np.float32, 1), 6: (np.float32, 1), 7: (np.uint32, 1), 8: (np.uint32, 1), 9: (np.uint32, 1),
     10: (np.uint32, 1), 11: (np.uint32, 1), 12: (np.uint32, 1), 13: (np.float32, 4), 14: (np.float32, 4),
+    15: (np.uint32, 1),
 }
 
 _f3 = C.c_float * 3
@@ -109,6 +112,10 @@ class PbdParams(C.Structure):
         ("grid_origin", _f3),
         ("grid_dims", _u3),
         ("tile_bits", C.c_uint32),
+        ("friction_static", C.c_float),
+        ("friction_kinetic", C.c_float),
+        ("box_solid_fraction", C.c_float),
+        ("box_solid_radius", C.c_float),
     ]
 
     def copy(self) -> "PbdParams":
@@ -143,6 +150,7 @@ _SIGS = {
     "pbd_generate_box": (C.c_int, [C.POINTER(PbdParams), C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                    C.POINTER(C.c_uint32)]),
     "pbd_lattice_rest_density": (C.c_int, [C.POINTER(PbdParams), C.POINTER(C.c_double)]),
+    "pbd_generate_phases": (C.c_int, [C.POINTER(PbdParams), C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "pbd_morton_encode": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
     "pbd_cell_key": (C.c_int, [C.POINTER(PbdParams), C.POINTER(C.c_float), C.POINTER(C.c_uint32),
                                C.POINTER(C.c_uint32)]),
@@ -283,6 +291,15 @@ def write_frame_arrays(path: str, pos4, col4=None, frame: int = 0, texture_width
         raise PbdError(rc, "pbd_write_frame_arrays: " + (lib().pbd_last_error(None) or b"").decode())
 
 
+def generate_phases(p: PbdParams) -> np.ndarray:
+    """Phases of the Particles.Box spawn (pbd_generate_phases): solid inside seeded spheres."""
+    n = p.box_width * p.box_height * p.box_depth
+    out = np.zeros(n, np.uint32)
+    got = C.c_uint32(0)
+    check(lib().pbd_generate_phases(C.byref(p), out.ctypes.data, n, C.byref(got)), "pbd_generate_phases")
+    return out
+
+
 def lattice_rest_density(p: PbdParams) -> float:
     out = C.c_double(0)
     check(lib().pbd_lattice_rest_density(C.byref(p), C.byref(out)), "pbd_lattice_rest_density")
@@ -386,7 +403,7 @@ class Context:
 
     def upload(self, which, arr: np.ndarray) -> None:
         w = BUF[which] if isinstance(which, str) else int(which)
-        a = np.ascontiguousarray(arr, np.float32)
+        a = np.ascontiguousarray(arr, _BUF_DTYPE[w][0])
         self._check(lib().pbd_upload(self._ptr, w, a.ctypes.data, a.nbytes), "pbd_upload")
 
     def check_finite(self) -> int:
