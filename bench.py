@@ -230,8 +230,9 @@ def adjoint_leg(F, frames, k_ckpt, seed=5, stabilization=None):
 
 
 def large_scene(F, steps, warmup, config="dam_break_16m"):
-    """The plain step at 16.7M particles (256^3 lattice), whose ~6 GB working set
-    exceeds the 256 MB Infinity Cache that holds most of the 1M scene."""
+    """The step at 16.7M particles (256^3 lattice), whose ~6 GB working set
+    exceeds the 256 MB Infinity Cache that holds most of the 1M scene; with
+    config mixed_16m, 30 % of them granular solids with friction."""
     frame = F.load_params(F.config_path(config))
     p = frame.copy()
     p.delta_t = frame.delta_t / frame.substeps
@@ -264,7 +265,10 @@ def large_scene(F, steps, warmup, config="dam_break_16m"):
         stages[k] = {"us_per_launch": us, "launches": launches,
                      "algorithmic_GBps": (b / (us * 1e-6) / 1e9) if us else 0.0,
                      "hbm_frac": (b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS) if us else 0.0}
-    return {"workload": f"{config}: {n}-particle dam break (256^3 lattice, spacing 0.05, 25.6x12.8x12.8 box), "
+    what = ("dam break" if not p.box_solid_fraction else
+            f"dam break, {100 * p.box_solid_fraction:.0f} % granular solid spheres of radius {p.box_solid_radius:g}, "
+            f"friction mu_s {p.friction_static:g} / mu_k {p.friction_kinetic:g}")
+    return {"workload": f"{config}: {n}-particle {what} (256^3 lattice, spacing 0.05, 25.6x12.8x12.8 box), "
                         f"steps {warmup + 1}-{warmup + steps}",
             "particles": n, "steps": steps, "particle_steps_per_s": n * steps / el,
             "ms_per_step": 1e3 * el / steps, "stages": stages, "nonfinite_particles": bad}
@@ -392,7 +396,10 @@ def main():
         v["algorithmic_GBps"] = (b / (v["us_per_launch"] * 1e-6) / 1e9) if v["us_per_launch"] else 0.0
     traffic = measured_traffic(dom)
     batched = None if args.no_batched else batched_vs_plain(F, args.scenes, args.batched_steps, 3)
-    large = None if args.no_large else large_scene(F, args.large_steps, 3)
+    large = mixed = None
+    if not args.no_large:
+        large = large_scene(F, args.large_steps, 3)
+        mixed = large_scene(F, args.large_steps, 3, config="mixed_16m")
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_leg(F, args.adjoint_frames, args.adjoint_checkpoint)
@@ -440,6 +447,7 @@ def main():
         "neighbour_lists": counters,
         "batched": batched,
         "large_scene": large,
+        "mixed_scene": mixed,
         "adjoint": adjoint,
     }
     if not args.no_cpu:
