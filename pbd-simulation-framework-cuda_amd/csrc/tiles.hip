@@ -298,10 +298,15 @@ template <>
 struct TileCap<6> {
   static constexpr int value = 2880;  // 4x4x4 blocks: 216 halo cells, ~1730 at rest density
 };
-// LDS entries: the halo, the sentinel at CAP and 3 more that a scan unrolled
-// by 4 may read past a cell range (their hits are masked off).
+// Scan unroll: candidates tested per group in nl_column.
+#ifndef PBD_SCAN_UNROLL
+#define PBD_SCAN_UNROLL 4
+#endif
+constexpr uint32_t kScanUnroll = PBD_SCAN_UNROLL;
+// LDS entries: the halo, the sentinel at CAP and kScanUnroll - 1 more that an
+// unrolled scan may read past a cell range (their hits are masked off).
 template <int BITS>
-constexpr int tile_lds() { return TileCap<BITS>::value + 4; }
+constexpr int tile_lds() { return TileCap<BITS>::value + (int)kScanUnroll; }
 
 // Copy the block's halo into LDS (block-local coordinates; .w = payload(i)) and
 // place the sentinel at slot CAP.
@@ -316,7 +321,7 @@ __device__ __forceinline__ void tile_load(const DevParams& P, const TileShared<B
     const float3 o = tile_offset<BITS>(P, c, ix0, iy0, iz0);
     s_q[i] = make_float4(q.x + o.x, q.y + o.y, q.z + o.z, payload(gi));
   }
-  if (threadIdx.x < 4) s_q[CAP + threadIdx.x] = make_float4(kFar, kFar, kFar, 0.0f);
+  if (threadIdx.x < kScanUnroll) s_q[CAP + threadIdx.x] = make_float4(kFar, kFar, kFar, 0.0f);
 }
 
 // tile_load plus the global index of every slot in s_gi (the sentinel slot CAP
@@ -333,7 +338,7 @@ __device__ __forceinline__ void tile_load_gi(const DevParams& P, const TileShare
     s_q[i] = make_float4(q.x + o.x, q.y + o.y, q.z + o.z, payload(gi));
     s_gi[i] = gi;
   }
-  if (threadIdx.x < 4) s_q[CAP + threadIdx.x] = make_float4(kFar, kFar, kFar, 0.0f);
+  if (threadIdx.x < kScanUnroll) s_q[CAP + threadIdx.x] = make_float4(kFar, kFar, kFar, 0.0f);
   if (threadIdx.x == 0) s_gi[CAP] = any;
 }
 
@@ -359,24 +364,26 @@ __device__ __forceinline__ void nl_visit(const NeighbourList& L, uint32_t g, uin
 }
 
 // In-range bits of one stencil column: bit k of (lo, hi) is set when slot a + k
-// (k < b - a <= 64) is within the list radius.  Unrolled by 4: a group may read
-// up to 3 slots past b (still inside the LDS array); their bits are masked off.
+// (k < b - a <= 64) is within the list radius.  Unrolled by kScanUnroll: a group
+// may read up to kScanUnroll - 1 slots past b (still inside the LDS array); their
+// bits are masked off.
 __device__ __forceinline__ void nl_column(const float4* s_q, uint32_t a, uint32_t b, float qx, float qy, float qz,
                                           float r2max, uint32_t& lo, uint32_t& hi) {
+  constexpr uint32_t U = kScanUnroll;
   lo = 0u;
   hi = 0u;
-  for (uint32_t k = 0; a + k < b; k += 4) {
-    float4 c[4];
+  for (uint32_t k = 0; a + k < b; k += U) {
+    float4 c[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) c[u] = s_q[a + k + u];
-    uint32_t g4 = 0u;
+    for (uint32_t u = 0; u < U; ++u) c[u] = s_q[a + k + u];
+    uint32_t g = 0u;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (uint32_t u = 0; u < U; ++u) {
       const float rx = qx - c[u].x, ry = qy - c[u].y, rz = qz - c[u].z;
-      g4 |= (rx * rx + ry * ry + rz * rz < r2max) ? (1u << u) : 0u;
+      g |= (rx * rx + ry * ry + rz * rz < r2max) ? (1u << u) : 0u;
     }
-    if (k < 32u) lo |= g4 << k;
-    else hi |= g4 << (k - 32u);
+    if (k < 32u) lo |= g << k;
+    else hi |= g << (k - 32u);
   }
   const uint32_t len = b - a;
   if (len < 32u) lo &= (1u << len) - 1u;
