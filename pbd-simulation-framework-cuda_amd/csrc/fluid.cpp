@@ -57,17 +57,17 @@ int Fluid::alloc_table() {
   const uint32_t ts = host_morton(params_.grid_dims[0] - 1u, params_.grid_dims[1] - 1u, params_.grid_dims[2] - 1u) + 1u;
   if (B_.cstart && ts == B_.table_size) return PBD_OK;
   if (B_.cstart) {
-    for (void* q : {(void*)B_.cstart, (void*)B_.cend, (void*)tile_flag_, (void*)tile_bcount_}) {
+    for (void* q : {(void*)B_.cstart, (void*)B_.cend, (void*)tile_flag_, (void*)tile_end_, (void*)tile_bcount_}) {
       (void)hipFree(q);
       allocations_.erase(std::remove(allocations_.begin(), allocations_.end(), q), allocations_.end());
     }
     B_.cstart = B_.cend = nullptr;
-    tile_flag_ = tile_bcount_ = nullptr;
+    tile_flag_ = tile_end_ = tile_bcount_ = nullptr;
   }
-  // tile flags for the smallest tile (2^5 cells) and the compaction's block counts
+  // tile starts and ends for the smallest tile (2^5 cells) and the compaction's block counts
   const uint32_t nt = ((ts - 1u) >> 5) + 1u;
   if (dev_alloc(allocations_, &B_.cstart, ts) || dev_alloc(allocations_, &B_.cend, ts) ||
-      dev_alloc(allocations_, &tile_flag_, nt) ||
+      dev_alloc(allocations_, &tile_flag_, nt) || dev_alloc(allocations_, &tile_end_, nt) ||
       dev_alloc(allocations_, &tile_bcount_, (nt + kTileCompactChunk - 1u) / kTileCompactChunk))
     return fail(PBD_ERR_NOMEM, "cell table allocation failed");
   B_.table_size = ts;
@@ -119,8 +119,8 @@ int Fluid::init(const pbd_params& p, uint32_t capacity) {
   rc |= dev_alloc(allocations_, &sort_.keys_out, c);
   rc |= dev_alloc(allocations_, &sort_.vals_out, c);
   rc |= dev_alloc(allocations_, &d_scratch_, 4);
-  rc |= dev_alloc(allocations_, &tile_list_, c);
-  rc |= dev_alloc(allocations_, &tile_count_, 1);
+  rc |= dev_alloc(allocations_, &tile_list_, (size_t)2 * c);
+  rc |= dev_alloc(allocations_, &tile_count_, 2);
   rc |= dev_alloc(allocations_, &B_.NL, c * 8);  // up to 64 slots (8 chunks) per particle
   rc |= dev_alloc(allocations_, &B_.NLC, c);
   B_.nl_cap = cap_;
@@ -136,7 +136,7 @@ int Fluid::init(const pbd_params& p, uint32_t capacity) {
     if (e == hipSuccess) e = hipMemsetAsync(q, 0, c * 4, stream_);
   if (e == hipSuccess) e = hipMemsetAsync(sort_.err, 0, 4, stream_);
   if (e == hipSuccess) e = hipMemsetAsync(d_scratch_, 0, 16, stream_);
-  if (e == hipSuccess) e = hipMemsetAsync(tile_count_, 0, 4, stream_);
+  if (e == hipSuccess) e = hipMemsetAsync(tile_count_, 0, 8, stream_);
   if (e == hipSuccess) e = hipMemsetAsync(B_.STATS, 0, PBD_COUNTER_COUNT * 8, stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
   if (hip_check(e, "initial zeroing")) return status_;
@@ -340,10 +340,11 @@ int Fluid::enqueue_substep() {
   uint32_t tgrid = 0, ntiles = 0;
   if (tiles) {
     ntiles = ((B_.table_size - 1u) >> bits) + 1u;
-    // a multiple of XCDs x tile chunk, so that tile_rank maps the grid onto [0, tgrid)
+    // large tiles + small tiles / 4 <= occupied blocks <= min(ntiles, n); a multiple
+    // of XCDs x tile chunk, so that tile_pick maps the grid onto [0, tgrid)
     tgrid = (std::min(ntiles, P.n) + kTileGridMultiple - 1u) / kTileGridMultiple * kTileGridMultiple;
   }
-  const TileList TL{tile_list_, tile_count_};
+  const TileList TL{tile_list_, tile_list_ + cap_, tile_count_};
   const size_t n16 = (size_t)P.n * 16, n4 = (size_t)P.n * 4;
   if (rec_) {  // the step's input state, for the reverse of the predict stage
     if (tape_copy(rec_->Xin, B_.X, n16) || tape_copy(rec_->Vin, B_.V, n16) || tape_copy(rec_->Fin, B_.F, n16))
@@ -358,8 +359,10 @@ int Fluid::enqueue_substep() {
   PBD_LAUNCH(ST_SORT, launch_sort(st, sort_, P.n, B_.keys_in, &keys_sorted_, &ids_sorted_));
   // 4-5. reorderStorage + computeCellInfo (Fluid.cu:29,33)
   PBD_LAUNCH(ST_REORDER, launch_reorder(st, tpb, P, B_, keys_sorted_, ids_sorted_, tiles ? (uint32_t)bits : 0u,
-                                        tile_flag_));
-  if (tiles) PBD_LAUNCH(ST_TILES, launch_tile_compact(st, ntiles, tile_flag_, tile_bcount_, tile_list_, tile_count_));
+                                        tile_flag_, tile_end_));
+  if (tiles)
+    PBD_LAUNCH(ST_TILES, launch_tile_compact(st, ntiles, tile_flag_, tile_end_, tile_bcount_, tile_list_,
+                                             tile_list_ + cap_, tile_count_));
   const uint32_t* keys = keys_sorted_;
   if (rec_) {
     const size_t tb = (size_t)B_.table_size * 4;

@@ -115,9 +115,10 @@ class Fluid : public Cuda_Computable {
   const uint32_t* keys_sorted_ = nullptr;
   const uint32_t* ids_sorted_ = nullptr;
   uint32_t* d_scratch_ = nullptr;  // small device scratch: [0] finite count, [1] unused
-  uint32_t* tile_list_ = nullptr;  // occupied Morton blocks of the step, ascending
-  uint32_t* tile_count_ = nullptr;
-  uint32_t* tile_flag_ = nullptr;    // per Morton block: occupied this step
+  uint32_t* tile_list_ = nullptr;   // occupied Morton blocks of the step: large tiles, then (at cap_) small ones
+  uint32_t* tile_count_ = nullptr;  // [0] large, [1] small tiles
+  uint32_t* tile_flag_ = nullptr;   // per Morton block: 1 + first sorted index, 0 if empty this step
+  uint32_t* tile_end_ = nullptr;    // per occupied Morton block: 1 + last sorted index
   uint32_t* tile_bcount_ = nullptr;  // compaction block counts
   uint32_t* batch_solved_ = nullptr;   // collisionType 1 (allocated on first use)
   unsigned long long* batch_min_ = nullptr;

@@ -60,7 +60,7 @@ __global__ void k_reorder(DevParams P, const uint32_t* __restrict__ keys_s, cons
                           const float4* __restrict__ Pold, const float4* __restrict__ C, float4* __restrict__ Q,
                           float4* __restrict__ QX, float4* __restrict__ VS, float4* __restrict__ CS,
                           uint32_t* __restrict__ cstart, uint32_t* __restrict__ cend, uint32_t tile_bits,
-                          uint32_t* __restrict__ tile_flag) {
+                          uint32_t* __restrict__ tile_start, uint32_t* __restrict__ tile_end) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.n) return;
   const uint32_t key = keys_s[s];
@@ -69,9 +69,14 @@ __global__ void k_reorder(DevParams P, const uint32_t* __restrict__ keys_s, cons
   if (key != prev) {
     cstart[key] = s;
     if (s) cend[prev] = s;
-    // first particle of an occupied Morton block of 2^tile_bits cells
-    if (tile_bits && (s == 0 || (key >> tile_bits) != (prev >> tile_bits))) tile_flag[key >> tile_bits] = 1u;
+    // first particle of an occupied Morton block of 2^tile_bits cells; the
+    // previous block ends here
+    if (tile_bits && (s == 0 || (key >> tile_bits) != (prev >> tile_bits))) {
+      tile_start[key >> tile_bits] = s + 1u;
+      if (s) tile_end[prev >> tile_bits] = s;
+    }
   }
+  if (tile_bits && s == P.n - 1u) tile_end[key >> tile_bits] = P.n;
   uint32_t j = ids_s[s];
   if (j >= P.n) j = P.n - 1;  // ids are a permutation of [0, n); the clamp only bounds a failed sort
   float4 x = X[j], v = V[j], p;
@@ -319,9 +324,10 @@ hipError_t launch_predict_keys(hipStream_t st, uint32_t tpb, const DevParams& P,
 }
 
 hipError_t launch_reorder(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,
-                          const uint32_t* keys_s, const uint32_t* ids_s, uint32_t tile_bits, uint32_t* tile_flag) {
+                          const uint32_t* keys_s, const uint32_t* ids_s, uint32_t tile_bits, uint32_t* tile_start,
+                          uint32_t* tile_end) {
   hipLaunchKernelGGL(k_reorder, dim3(blocks_for(P.n, tpb)), dim3(tpb), 0, st, P, keys_s, ids_s, B.X, B.V, B.F, B.P,
-                     B.C, B.Q[0], B.QX[0], B.VS, B.CS, B.cstart, B.cend, tile_bits, tile_flag);
+                     B.C, B.Q[0], B.QX[0], B.VS, B.CS, B.cstart, B.cend, tile_bits, tile_start, tile_end);
   return hipGetLastError();
 }
 

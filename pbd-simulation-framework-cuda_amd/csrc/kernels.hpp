@@ -46,31 +46,38 @@ struct StepBuffers {
 };
 
 // Occupied Morton blocks of the current step in Morton order (the reorder
-// kernel flags them, launch_tile_compact lists them).
+// kernel marks them, launch_tile_compact lists them).  Blocks of at most
+// kTileSmall particles are "small": four of them share one workgroup, one wave
+// each; the others take a whole workgroup.
 struct TileList {
-  const uint32_t* list;   // block ids (cell id >> bits), ascending
-  const uint32_t* count;  // number of entries
+  const uint32_t* large;  // block ids (cell id >> bits) of the large tiles, ascending
+  const uint32_t* small;  // block ids of the small tiles, ascending
+  const uint32_t* count;  // count[0] large tiles, count[1] small tiles
 };
 
+constexpr uint32_t kTileSmall = 64;           // particles of a small tile (one wave)
 constexpr uint32_t kTileCompactChunk = 4096;  // flags per block of the compaction
 #ifndef PBD_TILE_CHUNK
 #define PBD_TILE_CHUNK 4
 #endif
 constexpr uint32_t kTileChunk = PBD_TILE_CHUNK;        // consecutive tiles per XCD turn (tiles.hip tile_rank)
 constexpr uint32_t kTileGridMultiple = 8 * kTileChunk;  // tile grids are a multiple of (XCDs x tile chunk)
-// list / *count <- the indices of the set flags among flags[0, nflags), ascending;
-// bcount holds ceil(nflags / kTileCompactChunk) block counts.
-hipError_t launch_tile_compact(hipStream_t st, uint32_t nflags, const uint32_t* flags, uint32_t* bcount,
-                               uint32_t* list, uint32_t* count);
+// Splits the occupied blocks (start[b] != 0) into large[] and small[] by their
+// particle count end[b] - (start[b] - 1), both ascending; count[0..1] = their
+// lengths.  bcount holds ceil(nflags / kTileCompactChunk) packed block counts.
+hipError_t launch_tile_compact(hipStream_t st, uint32_t nflags, const uint32_t* start, const uint32_t* end,
+                               uint32_t* bcount, uint32_t* large, uint32_t* small, uint32_t* count);
 
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n);
 // cstart[0, ts) = UINT_MAX, cend[0, ts) = n, flags[0, nflags) = 0 (flags may be null)
 hipError_t launch_cell_reset(hipStream_t st, uint32_t* cstart, uint32_t* cend, uint32_t ts, uint32_t n,
                              uint32_t* flags, uint32_t nflags);
 hipError_t launch_predict_keys(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B);
-// tile_flag[b] = 1 for every occupied Morton block b of 2^tile_bits cells (tile_bits != 0)
+// For every occupied Morton block b of 2^tile_bits cells (tile_bits != 0):
+// tile_start[b] = 1 + its first sorted index, tile_end[b] = 1 + its last.
 hipError_t launch_reorder(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,
-                          const uint32_t* keys_s, const uint32_t* ids_s, uint32_t tile_bits, uint32_t* tile_flag);
+                          const uint32_t* keys_s, const uint32_t* ids_s, uint32_t tile_bits, uint32_t* tile_start,
+                          uint32_t* tile_end);
 hipError_t launch_lambda(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B, const float4* Q,
                          const uint32_t* keys_s, bool first);
 hipError_t launch_count(hipStream_t st, uint32_t tpb, const DevParams& P, const StepBuffers& B,

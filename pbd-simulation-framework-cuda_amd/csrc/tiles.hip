@@ -35,6 +35,25 @@ struct TileGeom {
 
 constexpr uint32_t kTileThreads = 256;
 
+// A team walks one tile: the whole workgroup for a large tile, one wave for a
+// small one.  A wave's LDS accesses complete in order, so a wave team needs no
+// hardware barrier, only one the compiler does not move memory accesses across.
+struct TeamBlock {
+  static constexpr uint32_t kSize = kTileThreads;
+  static constexpr uint32_t kLoadUnroll = 2;  // halo loads in flight per thread
+  static __device__ __forceinline__ uint32_t tid() { return threadIdx.x; }
+  static __device__ __forceinline__ void sync() { __syncthreads(); }
+};
+struct TeamWave {
+  static constexpr uint32_t kSize = 64;
+  static constexpr uint32_t kLoadUnroll = 4;
+  static __device__ __forceinline__ uint32_t tid() { return threadIdx.x & 63u; }
+  static __device__ __forceinline__ void sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+};
+
 template <int BITS>
 struct TileShared {
   uint32_t lstart[TileGeom<BITS>::NH + 1];  // LDS start of each halo cell; lstart[NH] = total
@@ -44,14 +63,16 @@ struct TileShared {
 };
 
 // Steps 1 of the tile: halo cell table, LDS offsets, own range.  Returns the
-// halo particle count.
-template <int BITS>
+// halo particle count.  Every thread of the team takes R halo cells; all their
+// cell-table loads are issued before the scan.
+template <int BITS, class Team>
 __device__ __forceinline__ uint32_t tile_setup(const DevParams& P, uint32_t blk, const uint32_t* __restrict__ cs,
                                                const uint32_t* __restrict__ ce, TileShared<BITS>& sh, int& ix0,
                                                int& iy0, int& iz0) {
   using G = TileGeom<BITS>;
-  static_assert(G::NH <= (int)kTileThreads, "one halo cell per thread");
-  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  constexpr uint32_t R = (G::NH + Team::kSize - 1u) / Team::kSize;
+  static_assert(Team::kSize == 64u || R == 1u, "a workgroup team takes one halo cell per thread");
+  const uint32_t tid = Team::tid(), lane = threadIdx.x & 63u;
   const uint32_t first = blk << BITS;
   ix0 = (int)compact10(first >> 2);
   iy0 = (int)compact10(first >> 1);
@@ -60,38 +81,62 @@ __device__ __forceinline__ uint32_t tile_setup(const DevParams& P, uint32_t blk,
     sh.g0 = 0xFFFFFFFFu;
     sh.g1 = 0u;
   }
-  __syncthreads();
-  uint32_t cnt = 0, st = 0;
-  if (tid < (uint32_t)G::NH) {
-    const int hx = (int)tid / G::SX, hy = ((int)tid / G::SY) % G::HY, hz = (int)tid % G::HZ;
-    const int cx = ix0 - 1 + hx, cy = iy0 - 1 + hy, cz = iz0 - 1 + hz;
-    if (cx >= 0 && cy >= 0 && cz >= 0 && cx < (int)P.dims[0] && cy < (int)P.dims[1] && cz < (int)P.dims[2]) {
-      const uint32_t k = morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
-      const uint32_t s0 = cs[k];
-      if (s0 != 0xFFFFFFFFu) {
-        st = s0;
-        cnt = ce[k] - s0;
-        if (hx >= 1 && hx <= G::EX && hy >= 1 && hy <= G::EY && hz >= 1 && hz <= G::EZ) {
-          atomicMin(&sh.g0, st);
-          atomicMax(&sh.g1, st + cnt);
+  Team::sync();
+  uint32_t cnt[R];
+#pragma unroll
+  for (uint32_t r = 0; r < R; ++r) {
+    const uint32_t c = r * Team::kSize + tid;
+    uint32_t n = 0, st = 0;
+    if (c < (uint32_t)G::NH) {
+      const int hx = (int)c / G::SX, hy = ((int)c / G::SY) % G::HY, hz = (int)c % G::HZ;
+      const int cx = ix0 - 1 + hx, cy = iy0 - 1 + hy, cz = iz0 - 1 + hz;
+      if (cx >= 0 && cy >= 0 && cz >= 0 && cx < (int)P.dims[0] && cy < (int)P.dims[1] && cz < (int)P.dims[2]) {
+        const uint32_t k = morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+        const uint32_t s0 = cs[k], e0 = ce[k];
+        if (s0 != 0xFFFFFFFFu) {
+          st = s0;
+          n = e0 - s0;
+          if (hx >= 1 && hx <= G::EX && hy >= 1 && hy <= G::EY && hz >= 1 && hz <= G::EZ) {
+            atomicMin(&sh.g0, st);
+            atomicMax(&sh.g1, st + n);
+          }
         }
       }
+      sh.gstart[c] = st;
     }
-    sh.gstart[tid] = st;
+    cnt[r] = n;
   }
-  uint32_t x = cnt;
+  if constexpr (Team::kSize == 64u) {
+    uint32_t carry = 0;
 #pragma unroll
-  for (uint32_t o = 1; o < 64u; o <<= 1) {
-    const uint32_t y = __shfl_up(x, o);
-    if (lane >= o) x += y;
+    for (uint32_t r = 0; r < R; ++r) {
+      uint32_t x = cnt[r];
+#pragma unroll
+      for (uint32_t o = 1; o < 64u; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+      }
+      const uint32_t c = r * 64u + tid;
+      if (c < (uint32_t)G::NH) sh.lstart[c] = carry + x - cnt[r];
+      carry += __shfl(x, 63);
+    }
+    if (tid == 0) sh.lstart[G::NH] = carry;
+  } else {
+    const uint32_t wave = tid >> 6;
+    uint32_t x = cnt[0];
+#pragma unroll
+    for (uint32_t o = 1; o < 64u; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63u) sh.wsum[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (uint32_t w = 0; w < wave; ++w) pre += sh.wsum[w];
+    if (tid < (uint32_t)G::NH) sh.lstart[tid] = pre + x - cnt[0];
+    if (tid == (uint32_t)G::NH - 1u) sh.lstart[G::NH] = pre + x;
   }
-  if (lane == 63u) sh.wsum[wave] = x;
-  __syncthreads();
-  uint32_t pre = 0;
-  for (uint32_t w = 0; w < wave; ++w) pre += sh.wsum[w];
-  if (tid < (uint32_t)G::NH) sh.lstart[tid] = pre + x - cnt;
-  if (tid == (uint32_t)G::NH - 1u) sh.lstart[G::NH] = pre + x;
-  __syncthreads();
+  Team::sync();
   return sh.lstart[G::NH];
 }
 
@@ -176,22 +221,45 @@ __device__ __forceinline__ uint32_t tile_count(const TileShared<BITS>& sh, int h
 constexpr uint32_t kXcds = 8;
 static_assert(kTileGridMultiple % (kXcds * kTileChunk) == 0, "tile grid multiple");
 
-// Rank in the tile list of this block's tile, or UINT32_MAX if the block has none.
-__device__ __forceinline__ uint32_t tile_rank(const TileList& T) {
+// The tile of this thread's team: 1 = large tile `blk` for the whole workgroup,
+// 2 = small tile `blk` for this wave, 0 = none.  Ranks below count[0] are large
+// tiles; rank count[0] + q takes small tiles 4q .. 4q + 3, one per wave.
+__device__ __forceinline__ int tile_pick(const TileList& T, uint32_t& blk) {
   const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
   const uint32_t r = ((k / kTileChunk) * kXcds + x) * kTileChunk + k % kTileChunk;
-  return r < *T.count ? r : 0xFFFFFFFFu;
+  const uint32_t nl = T.count[0];
+  if (r < nl) {
+    blk = T.large[r];
+    return 1;
+  }
+  const uint32_t i = (r - nl) * (kTileThreads / 64u) + (threadIdx.x >> 6);
+  if (i < T.count[1]) {
+    blk = T.small[i];
+    return 2;
+  }
+  return 0;
 }
 
 // ---------------------------------------------------------------------------
-// Tile-list compaction: a two-kernel stream compaction of the tile flags.
-__global__ __launch_bounds__(1024) void k_tile_count(uint32_t nflags, const uint32_t* __restrict__ flags,
+// Tile-list compaction: a two-kernel stream compaction of the occupied blocks
+// into the large and the small list.  Per-thread and per-block counts are
+// packed as large | small << 16 (a block of the compaction covers 4096 flags).
+__device__ __forceinline__ uint32_t tile_class(uint32_t b, uint32_t nflags, const uint32_t* __restrict__ start,
+                                               const uint32_t* __restrict__ end) {
+  if (b >= nflags) return 0u;
+  const uint32_t s = start[b];
+  if (s == 0u) return 0u;
+  return end[b] - (s - 1u) <= kTileSmall ? 0x10000u : 1u;
+}
+
+__global__ __launch_bounds__(1024) void k_tile_count(uint32_t nflags, const uint32_t* __restrict__ start,
+                                                     const uint32_t* __restrict__ end,
                                                      uint32_t* __restrict__ bcount) {
   __shared__ uint32_t s_w[16];
   const uint32_t base = blockIdx.x * kTileCompactChunk + threadIdx.x * 4u;
   uint32_t c = 0;
 #pragma unroll
-  for (uint32_t k = 0; k < 4; ++k) c += (base + k < nflags && flags[base + k]) ? 1u : 0u;
+  for (uint32_t k = 0; k < 4; ++k) c += tile_class(base + k, nflags, start, end);
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
   if ((threadIdx.x & 63u) == 0) s_w[threadIdx.x >> 6] = c;
@@ -203,31 +271,47 @@ __global__ __launch_bounds__(1024) void k_tile_count(uint32_t nflags, const uint
   }
 }
 
-__global__ __launch_bounds__(1024) void k_tile_emit(uint32_t nflags, const uint32_t* __restrict__ flags,
-                                                    const uint32_t* __restrict__ bcount, uint32_t* __restrict__ list,
-                                                    uint32_t* __restrict__ count) {
-  __shared__ uint32_t s_w[16];
-  __shared__ uint32_t s_off;
+__global__ __launch_bounds__(1024) void k_tile_emit(uint32_t nflags, const uint32_t* __restrict__ start,
+                                                    const uint32_t* __restrict__ end,
+                                                    const uint32_t* __restrict__ bcount, uint32_t* __restrict__ large,
+                                                    uint32_t* __restrict__ small, uint32_t* __restrict__ count) {
+  __shared__ uint32_t s_w[16][2];
+  __shared__ uint32_t s_off[2];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  // offset of this block: the counts of the blocks before it
-  uint32_t o = 0;
-  for (uint32_t i = tid; i < blockIdx.x; i += 1024u) o += bcount[i];
+  // offsets of this block: the counts of the blocks before it (unpacked: the
+  // totals may exceed 16 bits)
+  uint32_t ol = 0, os = 0;
+  for (uint32_t i = tid; i < blockIdx.x; i += 1024u) {
+    const uint32_t c = bcount[i];
+    ol += c & 0xFFFFu;
+    os += c >> 16;
+  }
 #pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) o += __shfl_xor(o, d);
-  if (lane == 0) s_w[wave] = o;
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t t = 0;
-    for (int w = 0; w < 16; ++w) t += s_w[w];
-    s_off = t;
+  for (int d = 32; d >= 1; d >>= 1) {
+    ol += __shfl_xor(ol, d);
+    os += __shfl_xor(os, d);
+  }
+  if (lane == 0) {
+    s_w[wave][0] = ol;
+    s_w[wave][1] = os;
   }
   __syncthreads();
-  // this thread's flags and their exclusive scan over the block
+  if (tid == 0) {
+    uint32_t tl = 0, ts = 0;
+    for (int w = 0; w < 16; ++w) {
+      tl += s_w[w][0];
+      ts += s_w[w][1];
+    }
+    s_off[0] = tl;
+    s_off[1] = ts;
+  }
+  __syncthreads();
+  // this thread's blocks and their exclusive scan over the compaction block (packed)
   const uint32_t base = blockIdx.x * kTileCompactChunk + tid * 4u;
   uint32_t f[4], c = 0;
 #pragma unroll
   for (uint32_t k = 0; k < 4; ++k) {
-    f[k] = (base + k < nflags && flags[base + k]) ? 1u : 0u;
+    f[k] = tile_class(base + k, nflags, start, end);
     c += f[k];
   }
   uint32_t x = c;
@@ -237,22 +321,29 @@ __global__ __launch_bounds__(1024) void k_tile_emit(uint32_t nflags, const uint3
     if (lane >= d) x += y;
   }
   __syncthreads();
-  if (lane == 63u) s_w[wave] = x;
+  if (lane == 63u) s_w[wave][0] = x;
   __syncthreads();
-  uint32_t pre = s_off;
-  for (uint32_t w = 0; w < wave; ++w) pre += s_w[w];
-  uint32_t pos = pre + x - c;
+  uint32_t pre = 0;
+  for (uint32_t w = 0; w < wave; ++w) pre += s_w[w][0];
+  const uint32_t ex = pre + x - c;
+  uint32_t pl = s_off[0] + (ex & 0xFFFFu), ps = s_off[1] + (ex >> 16);
 #pragma unroll
-  for (uint32_t k = 0; k < 4; ++k)
-    if (f[k]) list[pos++] = base + k;
-  if (blockIdx.x == gridDim.x - 1 && tid == 1023u) *count = pre + x;
+  for (uint32_t k = 0; k < 4; ++k) {
+    if (f[k] == 1u) large[pl++] = base + k;
+    else if (f[k]) small[ps++] = base + k;
+  }
+  if (blockIdx.x == gridDim.x - 1 && tid == 1023u) {
+    const uint32_t tot = pre + x;
+    count[0] = s_off[0] + (tot & 0xFFFFu);
+    count[1] = s_off[1] + (tot >> 16);
+  }
 }
 
-hipError_t launch_tile_compact(hipStream_t st, uint32_t nflags, const uint32_t* flags, uint32_t* bcount,
-                               uint32_t* list, uint32_t* count) {
+hipError_t launch_tile_compact(hipStream_t st, uint32_t nflags, const uint32_t* start, const uint32_t* end,
+                               uint32_t* bcount, uint32_t* large, uint32_t* small, uint32_t* count) {
   const uint32_t blocks = (nflags + kTileCompactChunk - 1u) / kTileCompactChunk;
-  hipLaunchKernelGGL(k_tile_count, dim3(blocks), dim3(1024), 0, st, nflags, flags, bcount);
-  hipLaunchKernelGGL(k_tile_emit, dim3(blocks), dim3(1024), 0, st, nflags, flags, bcount, list, count);
+  hipLaunchKernelGGL(k_tile_count, dim3(blocks), dim3(1024), 0, st, nflags, start, end, bcount);
+  hipLaunchKernelGGL(k_tile_emit, dim3(blocks), dim3(1024), 0, st, nflags, start, end, bcount, large, small, count);
   return hipGetLastError();
 }
 
@@ -293,53 +384,76 @@ struct TileCap;
 template <>
 struct TileCap<5> {
   static constexpr int value = 1920;  // 2x4x4 blocks: 144 halo cells, ~1150 particles at rest density
+  static constexpr int small = 420;   // a small tile's halo (one wave, a quarter of the LDS)
 };
 template <>
 struct TileCap<6> {
   static constexpr int value = 2880;  // 4x4x4 blocks: 216 halo cells, ~1730 at rest density
+  static constexpr int small = 630;
 };
 // Scan unroll: candidates tested per group in nl_column.
 #ifndef PBD_SCAN_UNROLL
 #define PBD_SCAN_UNROLL 4
 #endif
 constexpr uint32_t kScanUnroll = PBD_SCAN_UNROLL;
-// LDS entries: the halo, the sentinel at CAP and kScanUnroll - 1 more that an
-// unrolled scan may read past a cell range (their hits are masked off).
-template <int BITS>
-constexpr int tile_lds() { return TileCap<BITS>::value + (int)kScanUnroll; }
 
-// Copy the block's halo into LDS (block-local coordinates; .w = payload(i)) and
-// place the sentinel at slot CAP.
-template <int BITS, int CAP, class Payload>
+// LDS of one team: cell table, the halo (CAP entries, the sentinel at CAP and
+// kScanUnroll - 1 more that an unrolled scan may read past a cell range; their
+// hits are masked off) and, with GI, the global index of every slot.
+template <int BITS, int CAP, bool GI>
+struct TileRegion {
+  TileShared<BITS> sh;
+  float4 q[CAP + kScanUnroll];
+  uint32_t gi[GI ? CAP + 1 : 1];
+};
+// A workgroup's LDS: one large tile, or four small tiles in the same bytes.
+template <int BITS, bool GI>
+union TileLds {
+  TileRegion<BITS, TileCap<BITS>::value, GI> big;
+  TileRegion<BITS, TileCap<BITS>::small, GI> small[kTileThreads / 64u];
+};
+static_assert(sizeof(TileLds<5, false>) == sizeof(TileRegion<5, TileCap<5>::value, false>), "small tiles fit");
+static_assert(sizeof(TileLds<5, true>) == sizeof(TileRegion<5, TileCap<5>::value, true>), "small tiles fit");
+static_assert(sizeof(TileLds<6, false>) == sizeof(TileRegion<6, TileCap<6>::value, false>), "small tiles fit");
+static_assert(sizeof(TileLds<6, true>) == sizeof(TileRegion<6, TileCap<6>::value, true>), "small tiles fit");
+
+// Copy the block's halo into LDS (block-local coordinates; .w = payload(i)),
+// with s_gi (when not null) the global index of every slot, and place the
+// sentinel at slot CAP (s_gi[CAP] = `any`, a valid particle whose pair terms
+// the sentinel's position zeroes).  kLoadUnroll slots per thread are in flight
+// at once; the index of a slot past the end is clamped to the last slot and its
+// result dropped.
+template <int BITS, int CAP, class Team, class Payload>
 __device__ __forceinline__ void tile_load(const DevParams& P, const TileShared<BITS>& sh, uint32_t total,
                                           const float4* __restrict__ Q, int ix0, int iy0, int iz0, float4* s_q,
-                                          Payload payload) {
-  for (uint32_t i = threadIdx.x; i < total; i += kTileThreads) {
-    const int c = tile_cell_of<BITS>(sh, i);
-    const uint32_t gi = sh.gstart[c] + (i - sh.lstart[c]);
-    const float4 q = Q[gi];
-    const float3 o = tile_offset<BITS>(P, c, ix0, iy0, iz0);
-    s_q[i] = make_float4(q.x + o.x, q.y + o.y, q.z + o.z, payload(gi));
+                                          uint32_t* s_gi, uint32_t any, Payload payload) {
+  constexpr uint32_t U = Team::kLoadUnroll;
+  const uint32_t tid = Team::tid();
+  for (uint32_t i0 = tid; i0 < total; i0 += U * Team::kSize) {
+    int c[U];
+    uint32_t gi[U];
+    float4 q[U];
+    float w[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t i = min(i0 + u * Team::kSize, total - 1u);
+      c[u] = tile_cell_of<BITS>(sh, i);
+      gi[u] = sh.gstart[c[u]] + (i - sh.lstart[c[u]]);
+      q[u] = Q[gi[u]];
+      w[u] = payload(gi[u]);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t i = i0 + u * Team::kSize;
+      if (i < total) {
+        const float3 o = tile_offset<BITS>(P, c[u], ix0, iy0, iz0);
+        s_q[i] = make_float4(q[u].x + o.x, q[u].y + o.y, q[u].z + o.z, w[u]);
+        if (s_gi) s_gi[i] = gi[u];
+      }
+    }
   }
-  if (threadIdx.x < kScanUnroll) s_q[CAP + threadIdx.x] = make_float4(kFar, kFar, kFar, 0.0f);
-}
-
-// tile_load plus the global index of every slot in s_gi (the sentinel slot CAP
-// maps to `any`, a valid particle whose pair terms the sentinel's position zeroes).
-template <int BITS, int CAP, class Payload>
-__device__ __forceinline__ void tile_load_gi(const DevParams& P, const TileShared<BITS>& sh, uint32_t total,
-                                             const float4* __restrict__ Q, int ix0, int iy0, int iz0, float4* s_q,
-                                             uint32_t* s_gi, uint32_t any, Payload payload) {
-  for (uint32_t i = threadIdx.x; i < total; i += kTileThreads) {
-    const int c = tile_cell_of<BITS>(sh, i);
-    const uint32_t gi = sh.gstart[c] + (i - sh.lstart[c]);
-    const float4 q = Q[gi];
-    const float3 o = tile_offset<BITS>(P, c, ix0, iy0, iz0);
-    s_q[i] = make_float4(q.x + o.x, q.y + o.y, q.z + o.z, payload(gi));
-    s_gi[i] = gi;
-  }
-  if (threadIdx.x < kScanUnroll) s_q[CAP + threadIdx.x] = make_float4(kFar, kFar, kFar, 0.0f);
-  if (threadIdx.x == 0) s_gi[CAP] = any;
+  if (tid < kScanUnroll) s_q[CAP + tid] = make_float4(kFar, kFar, kFar, 0.0f);
+  if (s_gi && tid == 0) s_gi[CAP] = any;
 }
 
 // Visit the global list of particle g (n = padded length) in order.
@@ -454,35 +568,31 @@ __device__ __forceinline__ uint32_t nl_build(const DevParams& P, const Neighbour
   return cw.c * 8u;
 }
 
-template <int BITS, bool FIRST>
-__global__ __launch_bounds__(kTileThreads) void k_tile_lambda(DevParams P, TileList T, NeighbourList L,
-                                                              const float4* __restrict__ Q,
-                                                              const uint32_t* __restrict__ keys,
-                                                              const uint32_t* __restrict__ cs,
-                                                              const uint32_t* __restrict__ ce, float* __restrict__ lam,
-                                                              float* __restrict__ dens, uint32_t* __restrict__ ncnt,
-                                                              uint32_t* __restrict__ ccnt) {
+// lambda over one tile (team: workgroup or wave, CAP: its LDS capacity),
+// building the neighbour lists.
+template <int BITS, bool FIRST, class Team, int CAP>
+__device__ __forceinline__ void lambda_tile(const DevParams& P, uint32_t blk, const NeighbourList& L,
+                                            const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
+                                            const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+                                            float* __restrict__ lam, float* __restrict__ dens,
+                                            uint32_t* __restrict__ ncnt, uint32_t* __restrict__ ccnt,
+                                            TileShared<BITS>& sh, float4* s_q) {
   constexpr int VAR = PBD_KERNEL_TEXTBOOK;
-  constexpr int CAP = TileCap<BITS>::value;
-  __shared__ TileShared<BITS> sh;
-  __shared__ float4 s_q[tile_lds<BITS>()];
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_LIST_BUILDS], 1ull);
-  const uint32_t rank = tile_rank(T);
-  if (rank == 0xFFFFFFFFu) return;
+  const uint32_t tid = Team::tid();
   int ix0, iy0, iz0;
-  const uint32_t total = tile_setup<BITS>(P, T.list[rank], cs, ce, sh, ix0, iy0, iz0);
+  const uint32_t total = tile_setup<BITS, Team>(P, blk, cs, ce, sh, ix0, iy0, iz0);
   const uint32_t g0 = sh.g0, g1 = sh.g1;
   if (total > (uint32_t)CAP) {
-    for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
+    for (uint32_t g = g0 + tid; g < g1; g += Team::kSize) {
       lambda_particle<VAR, FIRST>(P, g, Q, keys, cs, ce, lam, dens, ncnt, ccnt);
       L.cnt[g] = kNlInvalid;
     }
-    if (threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_BLOCK], (unsigned long long)(g1 - g0));
+    if (tid == 0) atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_BLOCK], (unsigned long long)(g1 - g0));
     return;
   }
-  tile_load<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, [](uint32_t) { return 0.0f; });
-  __syncthreads();
-  for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
+  tile_load<BITS, CAP, Team>(P, sh, total, Q, ix0, iy0, iz0, s_q, nullptr, 0u, [](uint32_t) { return 0.0f; });
+  Team::sync();
+  for (uint32_t g = g0 + tid; g < g1; g += Team::kSize) {
     int h;
     uint32_t slot;
     tile_locate<BITS>(sh, keys[g], g, ix0, iy0, iz0, h, slot);
@@ -517,35 +627,51 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_lambda(DevParams P, TileL
   }
 }
 
-// Delta-p (+ collisions, apply, update) over the lambda pass's lists (use_list)
-// or a scan of the stencil.
-template <int BITS, bool COLLIDE, int NS>
-__global__ __launch_bounds__(kTileThreads) void k_tile_delta(DevParams P, TileList T, NeighbourList L, int use_list,
-                                                             uint32_t flags, const float4* __restrict__ Q,
-                                                             const float4* __restrict__ QX,
-                                                             const float* __restrict__ lam,
-                                                             const uint32_t* __restrict__ keys,
-                                                             const uint32_t* __restrict__ cs,
-                                                             const uint32_t* __restrict__ ce, float4* __restrict__ Qo,
-                                                             float4* __restrict__ QXo, float4* __restrict__ DP,
-                                                             float4* __restrict__ VS) {
+template <int BITS, bool FIRST>
+__global__ __launch_bounds__(kTileThreads) void k_tile_lambda(DevParams P, TileList T, NeighbourList L,
+                                                              const float4* __restrict__ Q,
+                                                              const uint32_t* __restrict__ keys,
+                                                              const uint32_t* __restrict__ cs,
+                                                              const uint32_t* __restrict__ ce, float* __restrict__ lam,
+                                                              float* __restrict__ dens, uint32_t* __restrict__ ncnt,
+                                                              uint32_t* __restrict__ ccnt) {
+  __shared__ TileLds<BITS, false> lds;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_LIST_BUILDS], 1ull);
+  uint32_t blk;
+  const int mode = tile_pick(T, blk);
+  if (mode == 1) {
+    lambda_tile<BITS, FIRST, TeamBlock, TileCap<BITS>::value>(P, blk, L, Q, keys, cs, ce, lam, dens, ncnt, ccnt,
+                                                              lds.big.sh, lds.big.q);
+  } else if (mode == 2) {
+    auto& r = lds.small[threadIdx.x >> 6];
+    lambda_tile<BITS, FIRST, TeamWave, TileCap<BITS>::small>(P, blk, L, Q, keys, cs, ce, lam, dens, ncnt, ccnt, r.sh,
+                                                             r.q);
+  }
+}
+
+// Delta-p (+ collisions, apply, update) over one tile, over the lambda pass's
+// lists (use_list) or a scan of the stencil.
+template <int BITS, bool COLLIDE, int NS, class Team, int CAP>
+__device__ __forceinline__ void delta_tile(const DevParams& P, uint32_t blk, const NeighbourList& L, int use_list,
+                                           uint32_t flags, const float4* __restrict__ Q,
+                                           const float4* __restrict__ QX, const float* __restrict__ lam,
+                                           const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
+                                           const uint32_t* __restrict__ ce, float4* __restrict__ Qo,
+                                           float4* __restrict__ QXo, float4* __restrict__ DP,
+                                           float4* __restrict__ VS, TileShared<BITS>& sh, float4* s_q) {
   constexpr int VAR = PBD_KERNEL_TEXTBOOK;
-  constexpr int CAP = TileCap<BITS>::value;
-  __shared__ TileShared<BITS> sh;
-  __shared__ float4 s_q[tile_lds<BITS>()];
-  const uint32_t rank = tile_rank(T);
-  if (rank == 0xFFFFFFFFu) return;
+  const uint32_t tid = Team::tid();
   int ix0, iy0, iz0;
-  const uint32_t total = tile_setup<BITS>(P, T.list[rank], cs, ce, sh, ix0, iy0, iz0);
+  const uint32_t total = tile_setup<BITS, Team>(P, blk, cs, ce, sh, ix0, iy0, iz0);
   const uint32_t g0 = sh.g0, g1 = sh.g1;
   if (total > (uint32_t)CAP) {
-    for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads)
+    for (uint32_t g = g0 + tid; g < g1; g += Team::kSize)
       delta_particle<VAR, true, COLLIDE>(P, flags, g, Q, QX, lam, keys, cs, ce, Qo, QXo, DP, VS);
     return;
   }
-  tile_load<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, [&](uint32_t gi) { return lam[gi]; });
-  __syncthreads();
-  for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
+  tile_load<BITS, CAP, Team>(P, sh, total, Q, ix0, iy0, iz0, s_q, nullptr, 0u, [&](uint32_t gi) { return lam[gi]; });
+  Team::sync();
+  for (uint32_t g = g0 + tid; g < g1; g += Team::kSize) {
     const uint32_t key = keys[g];
     int h;
     uint32_t slot;
@@ -570,40 +696,57 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_delta(DevParams P, TileLi
   }
 }
 
+template <int BITS, bool COLLIDE, int NS>
+__global__ __launch_bounds__(kTileThreads) void k_tile_delta(DevParams P, TileList T, NeighbourList L, int use_list,
+                                                             uint32_t flags, const float4* __restrict__ Q,
+                                                             const float4* __restrict__ QX,
+                                                             const float* __restrict__ lam,
+                                                             const uint32_t* __restrict__ keys,
+                                                             const uint32_t* __restrict__ cs,
+                                                             const uint32_t* __restrict__ ce, float4* __restrict__ Qo,
+                                                             float4* __restrict__ QXo, float4* __restrict__ DP,
+                                                             float4* __restrict__ VS) {
+  __shared__ TileLds<BITS, false> lds;
+  uint32_t blk;
+  const int mode = tile_pick(T, blk);
+  if (mode == 1) {
+    delta_tile<BITS, COLLIDE, NS, TeamBlock, TileCap<BITS>::value>(P, blk, L, use_list, flags, Q, QX, lam, keys, cs,
+                                                                   ce, Qo, QXo, DP, VS, lds.big.sh, lds.big.q);
+  } else if (mode == 2) {
+    auto& r = lds.small[threadIdx.x >> 6];
+    delta_tile<BITS, COLLIDE, NS, TeamWave, TileCap<BITS>::small>(P, blk, L, use_list, flags, Q, QX, lam, keys, cs,
+                                                                  ce, Qo, QXo, DP, VS, r.sh, r.q);
+  }
+}
+
 // XSPH: the same neighbour walk also applies XSPH (computeViscosity,
 // Density.h:280-310) and writes the velocities to Vo, in the order and with
 // the arithmetic of the final pass, which then leaves them alone.
-template <int BITS, bool XSPH>
-__global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileList T, NeighbourList L,
-                                                             const float4* __restrict__ Q,
-                                                             const float4* __restrict__ VS,
-                                                             const uint32_t* __restrict__ keys,
-                                                             const uint32_t* __restrict__ cs,
-                                                             const uint32_t* __restrict__ ce, float4* __restrict__ OM,
-                                                             float4* __restrict__ Vo) {
+template <int BITS, bool XSPH, class Team, int CAP>
+__device__ __forceinline__ void omega_tile(const DevParams& P, uint32_t blk, const NeighbourList& L,
+                                           const float4* __restrict__ Q, const float4* __restrict__ VS,
+                                           const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
+                                           const uint32_t* __restrict__ ce, float4* __restrict__ OM,
+                                           float4* __restrict__ Vo, TileShared<BITS>& sh, float4* s_q,
+                                           uint32_t* s_gi) {
   constexpr int VAR = PBD_KERNEL_TEXTBOOK;
-  constexpr int CAP = TileCap<BITS>::value;
-  __shared__ TileShared<BITS> sh;
-  __shared__ float4 s_q[tile_lds<BITS>()];
-  __shared__ uint32_t s_gi[CAP + 1];  // global index of each slot: velocities are gathered from L2
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_LIST_BUILDS], 1ull);
-  const uint32_t rank = tile_rank(T);
-  if (rank == 0xFFFFFFFFu) return;
+  const uint32_t tid = Team::tid();
   int ix0, iy0, iz0;
-  const uint32_t total = tile_setup<BITS>(P, T.list[rank], cs, ce, sh, ix0, iy0, iz0);
+  const uint32_t total = tile_setup<BITS, Team>(P, blk, cs, ce, sh, ix0, iy0, iz0);
   const uint32_t g0 = sh.g0, g1 = sh.g1;
   if (total > (uint32_t)CAP) {
-    for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
+    for (uint32_t g = g0 + tid; g < g1; g += Team::kSize) {
       omega_particle<VAR>(P, g, Q, VS, keys, cs, ce, OM);
       if (XSPH) Vo[g] = xsph_particle<VAR>(P, g, Q, VS, keys, cs, ce);
       L.cnt[g] = kNlInvalid;
     }
-    if (threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_BLOCK], (unsigned long long)(g1 - g0));
+    if (tid == 0) atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_BLOCK], (unsigned long long)(g1 - g0));
     return;
   }
-  tile_load_gi<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, s_gi, g0, [](uint32_t) { return 0.0f; });
-  __syncthreads();
-  for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
+  // global index of each slot: velocities are gathered from L2
+  tile_load<BITS, CAP, Team>(P, sh, total, Q, ix0, iy0, iz0, s_q, s_gi, g0, [](uint32_t) { return 0.0f; });
+  Team::sync();
+  for (uint32_t g = g0 + tid; g < g1; g += Team::kSize) {
     int h;
     uint32_t slot;
     tile_locate<BITS>(sh, keys[g], g, ix0, iy0, iz0, h, slot);
@@ -643,35 +786,56 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileLi
   }
 }
 
-template <int BITS, bool VORT, bool XSPH>
-__global__ __launch_bounds__(kTileThreads) void k_tile_final(
-    DevParams P, TileList T, NeighbourList L, int use_list, uint32_t flags, const float4* __restrict__ Q,
-    const float4* __restrict__ QX, const float4* __restrict__ VS, const float4* __restrict__ OM,
-    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
-    const float4* __restrict__ CS, const uint32_t* __restrict__ ncnt, float4* __restrict__ Xo,
-    float4* __restrict__ Po, float4* __restrict__ Vo, float4* __restrict__ Co, float4* __restrict__ Fo) {
+// XSPH: the same neighbour walk also applies XSPH (computeViscosity,
+// Density.h:280-310) and writes the velocities to Vo, in the order and with
+// the arithmetic of the final pass, which then leaves them alone.
+template <int BITS, bool XSPH>
+__global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileList T, NeighbourList L,
+                                                             const float4* __restrict__ Q,
+                                                             const float4* __restrict__ VS,
+                                                             const uint32_t* __restrict__ keys,
+                                                             const uint32_t* __restrict__ cs,
+                                                             const uint32_t* __restrict__ ce, float4* __restrict__ OM,
+                                                             float4* __restrict__ Vo) {
+  __shared__ TileLds<BITS, true> lds;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_LIST_BUILDS], 1ull);
+  uint32_t blk;
+  const int mode = tile_pick(T, blk);
+  if (mode == 1) {
+    omega_tile<BITS, XSPH, TeamBlock, TileCap<BITS>::value>(P, blk, L, Q, VS, keys, cs, ce, OM, Vo, lds.big.sh,
+                                                            lds.big.q, lds.big.gi);
+  } else if (mode == 2) {
+    auto& r = lds.small[threadIdx.x >> 6];
+    omega_tile<BITS, XSPH, TeamWave, TileCap<BITS>::small>(P, blk, L, Q, VS, keys, cs, ce, OM, Vo, r.sh, r.q, r.gi);
+  }
+}
+
+template <int BITS, bool VORT, bool XSPH, class Team, int CAP>
+__device__ __forceinline__ void final_tile(const DevParams& P, uint32_t blk, const NeighbourList& L, int use_list,
+                                           uint32_t flags, const float4* __restrict__ Q,
+                                           const float4* __restrict__ QX, const float4* __restrict__ VS,
+                                           const float4* __restrict__ OM, const uint32_t* __restrict__ keys,
+                                           const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+                                           const float4* __restrict__ CS, const uint32_t* __restrict__ ncnt,
+                                           float4* __restrict__ Xo, float4* __restrict__ Po, float4* __restrict__ Vo,
+                                           float4* __restrict__ Co, float4* __restrict__ Fo, TileShared<BITS>& sh,
+                                           float4* s_q, uint32_t* s_gi) {
   constexpr int VAR = PBD_KERNEL_TEXTBOOK;
-  constexpr int CAP = TileCap<BITS>::value;
-  __shared__ TileShared<BITS> sh;
-  __shared__ float4 s_q[tile_lds<BITS>()];
-  // global index of each slot, for XSPH's velocity gathers from L2
-  __shared__ uint32_t s_gi[XSPH ? CAP + 1 : 1];
-  const uint32_t rank = tile_rank(T);
-  if (rank == 0xFFFFFFFFu) return;
+  const uint32_t tid = Team::tid();
   int ix0, iy0, iz0;
-  const uint32_t total = tile_setup<BITS>(P, T.list[rank], cs, ce, sh, ix0, iy0, iz0);
+  const uint32_t total = tile_setup<BITS, Team>(P, blk, cs, ce, sh, ix0, iy0, iz0);
   const uint32_t g0 = sh.g0, g1 = sh.g1;
   constexpr bool vort = VORT, xsph = XSPH;  // flags & FF_VORT, flags & FF_XSPH
   if (total > (uint32_t)CAP) {
-    for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads)
+    for (uint32_t g = g0 + tid; g < g1; g += Team::kSize)
       final_particle<VAR>(P, flags, g, Q, QX, VS, OM, keys, cs, ce, CS, ncnt, Xo, Po, Vo, Co, Fo);
     return;
   }
+  // global index of each slot (XSPH only), for the velocity gathers from L2
   const auto omega_len = [&](uint32_t gi) { return vort ? OM[gi].w : 0.0f; };
-  if (XSPH) tile_load_gi<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, s_gi, g0, omega_len);
-  else tile_load<BITS, CAP>(P, sh, total, Q, ix0, iy0, iz0, s_q, omega_len);
-  __syncthreads();
-  for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
+  tile_load<BITS, CAP, Team>(P, sh, total, Q, ix0, iy0, iz0, s_q, XSPH ? s_gi : nullptr, g0, omega_len);
+  Team::sync();
+  for (uint32_t g = g0 + tid; g < g1; g += Team::kSize) {
     const uint32_t key = keys[g];
     int h;
     uint32_t slot;
@@ -703,6 +867,27 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_final(
     const float cz = cell_corner(P.origin[2], P.cell, compact10(key));
     final_finish<VAR>(P, flags, g, Q[g], make_float4(vx, vy, vz, vw), oi, a, cx, cy, cz, QX, CS, ncnt, Xo, Po, Vo,
                       Co, Fo);
+  }
+}
+
+template <int BITS, bool VORT, bool XSPH>
+__global__ __launch_bounds__(kTileThreads) void k_tile_final(
+    DevParams P, TileList T, NeighbourList L, int use_list, uint32_t flags, const float4* __restrict__ Q,
+    const float4* __restrict__ QX, const float4* __restrict__ VS, const float4* __restrict__ OM,
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+    const float4* __restrict__ CS, const uint32_t* __restrict__ ncnt, float4* __restrict__ Xo,
+    float4* __restrict__ Po, float4* __restrict__ Vo, float4* __restrict__ Co, float4* __restrict__ Fo) {
+  __shared__ TileLds<BITS, XSPH> lds;
+  uint32_t blk;
+  const int mode = tile_pick(T, blk);
+  if (mode == 1) {
+    final_tile<BITS, VORT, XSPH, TeamBlock, TileCap<BITS>::value>(P, blk, L, use_list, flags, Q, QX, VS, OM, keys, cs,
+                                                                  ce, CS, ncnt, Xo, Po, Vo, Co, Fo, lds.big.sh,
+                                                                  lds.big.q, lds.big.gi);
+  } else if (mode == 2) {
+    auto& r = lds.small[threadIdx.x >> 6];
+    final_tile<BITS, VORT, XSPH, TeamWave, TileCap<BITS>::small>(P, blk, L, use_list, flags, Q, QX, VS, OM, keys, cs,
+                                                                 ce, CS, ncnt, Xo, Po, Vo, Co, Fo, r.sh, r.q, r.gi);
   }
 }
 
