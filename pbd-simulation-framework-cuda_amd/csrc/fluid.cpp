@@ -338,11 +338,14 @@ int Fluid::enqueue_substep() {
   const int bits = (int)params_.tile_bits;
   const bool tiles = bits != 0 && P.variant == PBD_KERNEL_TEXTBOOK && P.kw == 1 && (g & G_FIND_CONTACTS) && !P.phased;
   uint32_t tgrid = 0, ntiles = 0;
+  bool tstride = false;
   if (tiles) {
     ntiles = ((B_.table_size - 1u) >> bits) + 1u;
-    // large tiles + small tiles / 4 <= occupied blocks <= min(ntiles, n); a multiple
-    // of XCDs x tile chunk, so that tile_pick maps the grid onto [0, tgrid)
+    // work items (large tiles + small tiles / 4) <= occupied blocks <= min(ntiles, n);
+    // a multiple of XCDs x tile chunk, so that tile_rank0 maps the grid onto [0, tgrid)
     tgrid = (std::min(ntiles, P.n) + kTileGridMultiple - 1u) / kTileGridMultiple * kTileGridMultiple;
+    tstride = tgrid > kTileGridDirect;
+    if (tstride) tgrid = kTileGridStride;
   }
   const TileList TL{tile_list_, tile_list_ + cap_, tile_count_};
   const size_t n16 = (size_t)P.n * 16, n4 = (size_t)P.n * 4;
@@ -387,7 +390,7 @@ int Fluid::enqueue_substep() {
     bool listed = false;  // the lambda pass left the neighbour lists of Q[cur]
     if (lambda_on) {
       if (tiles) {
-        PBD_LAUNCH(ST_LAMBDA, launch_tile_lambda(st, bits, tgrid, P, TL, B_, B_.Q[cur], keys, !counted));
+        PBD_LAUNCH(ST_LAMBDA, launch_tile_lambda(st, bits, tgrid, tstride, P, TL, B_, B_.Q[cur], keys, !counted));
         listed = true;
       } else {
         PBD_LAUNCH(ST_LAMBDA, launch_lambda(st, tpb, P, B_, B_.Q[cur], keys, !counted));
@@ -434,7 +437,7 @@ int Fluid::enqueue_substep() {
       }
       const int stage = ks[k].delta ? ST_DELTA : ST_COLLIDE;
       if (tiles && ks[k].delta) {
-        PBD_LAUNCH(stage, launch_tile_delta(st, bits, tgrid, P, TL, B_, fl, ks[k].collide, listed && k == 0,
+        PBD_LAUNCH(stage, launch_tile_delta(st, bits, tgrid, tstride, P, TL, B_, fl, ks[k].collide, listed && k == 0,
                                             B_.Q[cur], B_.QX[cur], B_.Q[1 - cur], B_.QX[1 - cur], keys));
       } else {
         PBD_LAUNCH(stage, launch_delta(st, tpb, P, B_, fl, ks[k].delta, ks[k].collide, B_.Q[cur], B_.QX[cur],
@@ -456,7 +459,7 @@ int Fluid::enqueue_substep() {
   const bool xsph_in_omega = omega_listed && (g & G_VISCOSITY);
   if (g & G_OMEGA) {
     if (tiles) {
-      PBD_LAUNCH(ST_OMEGA, launch_tile_omega(st, bits, tgrid, P, TL, B_, B_.Q[cur], keys, xsph_in_omega));
+      PBD_LAUNCH(ST_OMEGA, launch_tile_omega(st, bits, tgrid, tstride, P, TL, B_, B_.Q[cur], keys, xsph_in_omega));
     } else {
       PBD_LAUNCH(ST_OMEGA, launch_omega(st, tpb, P, B_, B_.Q[cur], keys));
     }
@@ -468,7 +471,7 @@ int Fluid::enqueue_substep() {
   if (g & G_VISCOSITY) ff |= xsph_in_omega ? FF_XSPH_DONE : FF_XSPH;
   if (updated) ff |= FF_UPDATED;
   if (tiles) {
-    PBD_LAUNCH(ST_FINAL, launch_tile_final(st, bits, tgrid, P, TL, B_, ff, omega_listed, B_.Q[cur], B_.QX[cur], keys));
+    PBD_LAUNCH(ST_FINAL, launch_tile_final(st, bits, tgrid, tstride, P, TL, B_, ff, omega_listed, B_.Q[cur], B_.QX[cur], keys));
   } else
     PBD_LAUNCH(ST_FINAL, launch_final(st, tpb, P, B_, ff, B_.Q[cur], B_.QX[cur], keys));
   return PBD_OK;

@@ -60,8 +60,15 @@ constexpr uint32_t kTileCompactChunk = 4096;  // flags per block of the compacti
 #ifndef PBD_TILE_CHUNK
 #define PBD_TILE_CHUNK 4
 #endif
-constexpr uint32_t kTileChunk = PBD_TILE_CHUNK;        // consecutive tiles per XCD turn (tiles.hip tile_rank)
+constexpr uint32_t kTileChunk = PBD_TILE_CHUNK;        // consecutive tiles per XCD turn (tiles.hip tile_rank0)
 constexpr uint32_t kTileGridMultiple = 8 * kTileChunk;  // tile grids are a multiple of (XCDs x tile chunk)
+// A tile grid covers the work items (one per workgroup) up to kTileGridDirect
+// workgroups.  Above that (a batch's cell table bounds the items far above the
+// occupied blocks) the passes run on kTileGridStride workgroups, each taking
+// every gridDim-th item.
+constexpr uint32_t kTileGridDirect = 1u << 18;
+constexpr uint32_t kTileGridStride = 16384;
+static_assert(kTileGridStride % kTileGridMultiple == 0 && kTileGridDirect % kTileGridMultiple == 0, "tile grids");
 // Splits the occupied blocks (start[b] != 0) into large[] and small[] by their
 // particle count end[b] - (start[b] - 1), both ascending; count[0..1] = their
 // lengths.  bcount holds ceil(nflags / kTileCompactChunk) packed block counts.
@@ -101,15 +108,15 @@ hipError_t launch_check_finite(hipStream_t st, uint32_t n, const float4* X, cons
 // ---- LDS cell-block tiles (tiles.hip), textbook variant, 27-cell stencil ----
 // The lambda and omega passes write the neighbour lists that the following
 // delta-p and final passes read when use_list is set.
-hipError_t launch_tile_lambda(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
+hipError_t launch_tile_lambda(hipStream_t st, int bits, uint32_t grid, bool stride, const DevParams& P, const TileList& T,
                               const StepBuffers& B, const float4* Q, const uint32_t* keys, bool first);
-hipError_t launch_tile_delta(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
+hipError_t launch_tile_delta(hipStream_t st, int bits, uint32_t grid, bool stride, const DevParams& P, const TileList& T,
                              const StepBuffers& B, uint32_t flags, bool collide, bool use_list, const float4* Q,
                              const float4* QX, float4* Qo, float4* QXo, const uint32_t* keys);
 // xsph: also apply XSPH (computeViscosity) and write the velocities to B.V
-hipError_t launch_tile_omega(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
+hipError_t launch_tile_omega(hipStream_t st, int bits, uint32_t grid, bool stride, const DevParams& P, const TileList& T,
                              const StepBuffers& B, const float4* Q, const uint32_t* keys, bool xsph);
-hipError_t launch_tile_final(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
+hipError_t launch_tile_final(hipStream_t st, int bits, uint32_t grid, bool stride, const DevParams& P, const TileList& T,
                              const StepBuffers& B, uint32_t flags, bool use_list, const float4* Q, const float4* QX,
                              const uint32_t* keys);
 

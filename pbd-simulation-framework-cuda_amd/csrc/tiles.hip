@@ -221,19 +221,32 @@ __device__ __forceinline__ uint32_t tile_count(const TileShared<BITS>& sh, int h
 constexpr uint32_t kXcds = 8;
 static_assert(kTileGridMultiple % (kXcds * kTileChunk) == 0, "tile grid multiple");
 
-// The tile of this thread's team: 1 = large tile `blk` for the whole workgroup,
-// 2 = small tile `blk` for this wave, 0 = none.  Ranks below count[0] are large
-// tiles; rank count[0] + q takes small tiles 4q .. 4q + 3, one per wave.
-__device__ __forceinline__ int tile_pick(const TileList& T, uint32_t& blk) {
+// Work items of the tile passes: ranks below nl are large tiles; rank nl + q
+// takes small tiles 4q .. 4q + 3, one per wave.  A workgroup takes the ranks
+// tile_rank0() + k gridDim.x (k = 0, 1, ...): the grid may be smaller than the
+// number of items, and any grid size covers them all.
+struct TileItems {
+  uint32_t nl, ns, end;  // large tiles, small tiles, ranks
+};
+__device__ __forceinline__ TileItems tile_items(const TileList& T) {
+  const uint32_t nl = T.count[0], ns = T.count[1];
+  return TileItems{nl, ns, nl + (ns + 3u) / 4u};
+}
+__device__ __forceinline__ uint32_t tile_rank0() {
   const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
-  const uint32_t r = ((k / kTileChunk) * kXcds + x) * kTileChunk + k % kTileChunk;
-  const uint32_t nl = T.count[0];
-  if (r < nl) {
+  return ((k / kTileChunk) * kXcds + x) * kTileChunk + k % kTileChunk;
+}
+// The tile of rank r for this thread's team: 1 = large tile `blk` for the whole
+// workgroup, 2 = small tile `blk` for this wave, 0 = none (for this wave, none at
+// any later rank either).
+__device__ __forceinline__ int tile_pick(const TileList& T, const TileItems& it, uint32_t r, uint32_t& blk) {
+  if (r < it.nl) {
     blk = T.large[r];
     return 1;
   }
-  const uint32_t i = (r - nl) * (kTileThreads / 64u) + (threadIdx.x >> 6);
-  if (i < T.count[1]) {
+  if (r >= it.end) return 0;
+  const uint32_t i = (r - it.nl) * (kTileThreads / 64u) + (threadIdx.x >> 6);
+  if (i < it.ns) {
     blk = T.small[i];
     return 2;
   }
@@ -627,7 +640,7 @@ __device__ __forceinline__ void lambda_tile(const DevParams& P, uint32_t blk, co
   }
 }
 
-template <int BITS, bool FIRST>
+template <int BITS, bool FIRST, bool STRIDE>
 __global__ __launch_bounds__(kTileThreads) void k_tile_lambda(DevParams P, TileList T, NeighbourList L,
                                                               const float4* __restrict__ Q,
                                                               const uint32_t* __restrict__ keys,
@@ -637,15 +650,24 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_lambda(DevParams P, TileL
                                                               uint32_t* __restrict__ ccnt) {
   __shared__ TileLds<BITS, false> lds;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_LIST_BUILDS], 1ull);
-  uint32_t blk;
-  const int mode = tile_pick(T, blk);
-  if (mode == 1) {
-    lambda_tile<BITS, FIRST, TeamBlock, TileCap<BITS>::value>(P, blk, L, Q, keys, cs, ce, lam, dens, ncnt, ccnt,
-                                                              lds.big.sh, lds.big.q);
-  } else if (mode == 2) {
-    auto& r = lds.small[threadIdx.x >> 6];
-    lambda_tile<BITS, FIRST, TeamWave, TileCap<BITS>::small>(P, blk, L, Q, keys, cs, ce, lam, dens, ncnt, ccnt, r.sh,
-                                                             r.q);
+  const TileItems it = tile_items(T);
+  // STRIDE: every gridDim-th rank from tile_rank0(); otherwise the grid covers
+  // the ranks and a workgroup takes one
+  for (uint32_t r = tile_rank0();; r += gridDim.x) {
+    uint32_t blk;
+    const int mode = tile_pick(T, it, r, blk);
+    if (mode == 0) break;
+    if (mode == 1) {
+      lambda_tile<BITS, FIRST, TeamBlock, TileCap<BITS>::value>(P, blk, L, Q, keys, cs, ce, lam, dens, ncnt, ccnt,
+                                                                lds.big.sh, lds.big.q);
+      if (!STRIDE) break;
+      __syncthreads();  // the next rank may be small tiles in the same LDS bytes
+    } else {
+      auto& reg = lds.small[threadIdx.x >> 6];
+      lambda_tile<BITS, FIRST, TeamWave, TileCap<BITS>::small>(P, blk, L, Q, keys, cs, ce, lam, dens, ncnt, ccnt, reg.sh,
+                                                               reg.q);
+      if (!STRIDE) break;
+    }
   }
 }
 
@@ -696,7 +718,7 @@ __device__ __forceinline__ void delta_tile(const DevParams& P, uint32_t blk, con
   }
 }
 
-template <int BITS, bool COLLIDE, int NS>
+template <int BITS, bool COLLIDE, int NS, bool STRIDE>
 __global__ __launch_bounds__(kTileThreads) void k_tile_delta(DevParams P, TileList T, NeighbourList L, int use_list,
                                                              uint32_t flags, const float4* __restrict__ Q,
                                                              const float4* __restrict__ QX,
@@ -707,15 +729,24 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_delta(DevParams P, TileLi
                                                              float4* __restrict__ QXo, float4* __restrict__ DP,
                                                              float4* __restrict__ VS) {
   __shared__ TileLds<BITS, false> lds;
-  uint32_t blk;
-  const int mode = tile_pick(T, blk);
-  if (mode == 1) {
-    delta_tile<BITS, COLLIDE, NS, TeamBlock, TileCap<BITS>::value>(P, blk, L, use_list, flags, Q, QX, lam, keys, cs,
-                                                                   ce, Qo, QXo, DP, VS, lds.big.sh, lds.big.q);
-  } else if (mode == 2) {
-    auto& r = lds.small[threadIdx.x >> 6];
-    delta_tile<BITS, COLLIDE, NS, TeamWave, TileCap<BITS>::small>(P, blk, L, use_list, flags, Q, QX, lam, keys, cs,
-                                                                  ce, Qo, QXo, DP, VS, r.sh, r.q);
+  const TileItems it = tile_items(T);
+  // STRIDE: every gridDim-th rank from tile_rank0(); otherwise the grid covers
+  // the ranks and a workgroup takes one
+  for (uint32_t r = tile_rank0();; r += gridDim.x) {
+    uint32_t blk;
+    const int mode = tile_pick(T, it, r, blk);
+    if (mode == 0) break;
+    if (mode == 1) {
+      delta_tile<BITS, COLLIDE, NS, TeamBlock, TileCap<BITS>::value>(P, blk, L, use_list, flags, Q, QX, lam, keys, cs,
+                                                                     ce, Qo, QXo, DP, VS, lds.big.sh, lds.big.q);
+      if (!STRIDE) break;
+      __syncthreads();  // the next rank may be small tiles in the same LDS bytes
+    } else {
+      auto& reg = lds.small[threadIdx.x >> 6];
+      delta_tile<BITS, COLLIDE, NS, TeamWave, TileCap<BITS>::small>(P, blk, L, use_list, flags, Q, QX, lam, keys, cs,
+                                                                    ce, Qo, QXo, DP, VS, reg.sh, reg.q);
+      if (!STRIDE) break;
+    }
   }
 }
 
@@ -789,7 +820,7 @@ __device__ __forceinline__ void omega_tile(const DevParams& P, uint32_t blk, con
 // XSPH: the same neighbour walk also applies XSPH (computeViscosity,
 // Density.h:280-310) and writes the velocities to Vo, in the order and with
 // the arithmetic of the final pass, which then leaves them alone.
-template <int BITS, bool XSPH>
+template <int BITS, bool XSPH, bool STRIDE>
 __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileList T, NeighbourList L,
                                                              const float4* __restrict__ Q,
                                                              const float4* __restrict__ VS,
@@ -799,14 +830,23 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileLi
                                                              float4* __restrict__ Vo) {
   __shared__ TileLds<BITS, true> lds;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&L.stats[PBD_COUNTER_LIST_BUILDS], 1ull);
-  uint32_t blk;
-  const int mode = tile_pick(T, blk);
-  if (mode == 1) {
-    omega_tile<BITS, XSPH, TeamBlock, TileCap<BITS>::value>(P, blk, L, Q, VS, keys, cs, ce, OM, Vo, lds.big.sh,
-                                                            lds.big.q, lds.big.gi);
-  } else if (mode == 2) {
-    auto& r = lds.small[threadIdx.x >> 6];
-    omega_tile<BITS, XSPH, TeamWave, TileCap<BITS>::small>(P, blk, L, Q, VS, keys, cs, ce, OM, Vo, r.sh, r.q, r.gi);
+  const TileItems it = tile_items(T);
+  // STRIDE: every gridDim-th rank from tile_rank0(); otherwise the grid covers
+  // the ranks and a workgroup takes one
+  for (uint32_t r = tile_rank0();; r += gridDim.x) {
+    uint32_t blk;
+    const int mode = tile_pick(T, it, r, blk);
+    if (mode == 0) break;
+    if (mode == 1) {
+      omega_tile<BITS, XSPH, TeamBlock, TileCap<BITS>::value>(P, blk, L, Q, VS, keys, cs, ce, OM, Vo, lds.big.sh,
+                                                              lds.big.q, lds.big.gi);
+      if (!STRIDE) break;
+      __syncthreads();  // the next rank may be small tiles in the same LDS bytes
+    } else {
+      auto& reg = lds.small[threadIdx.x >> 6];
+      omega_tile<BITS, XSPH, TeamWave, TileCap<BITS>::small>(P, blk, L, Q, VS, keys, cs, ce, OM, Vo, reg.sh, reg.q, reg.gi);
+      if (!STRIDE) break;
+    }
   }
 }
 
@@ -870,7 +910,7 @@ __device__ __forceinline__ void final_tile(const DevParams& P, uint32_t blk, con
   }
 }
 
-template <int BITS, bool VORT, bool XSPH>
+template <int BITS, bool VORT, bool XSPH, bool STRIDE>
 __global__ __launch_bounds__(kTileThreads) void k_tile_final(
     DevParams P, TileList T, NeighbourList L, int use_list, uint32_t flags, const float4* __restrict__ Q,
     const float4* __restrict__ QX, const float4* __restrict__ VS, const float4* __restrict__ OM,
@@ -878,16 +918,25 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_final(
     const float4* __restrict__ CS, const uint32_t* __restrict__ ncnt, float4* __restrict__ Xo,
     float4* __restrict__ Po, float4* __restrict__ Vo, float4* __restrict__ Co, float4* __restrict__ Fo) {
   __shared__ TileLds<BITS, XSPH> lds;
-  uint32_t blk;
-  const int mode = tile_pick(T, blk);
-  if (mode == 1) {
-    final_tile<BITS, VORT, XSPH, TeamBlock, TileCap<BITS>::value>(P, blk, L, use_list, flags, Q, QX, VS, OM, keys, cs,
-                                                                  ce, CS, ncnt, Xo, Po, Vo, Co, Fo, lds.big.sh,
-                                                                  lds.big.q, lds.big.gi);
-  } else if (mode == 2) {
-    auto& r = lds.small[threadIdx.x >> 6];
-    final_tile<BITS, VORT, XSPH, TeamWave, TileCap<BITS>::small>(P, blk, L, use_list, flags, Q, QX, VS, OM, keys, cs,
-                                                                 ce, CS, ncnt, Xo, Po, Vo, Co, Fo, r.sh, r.q, r.gi);
+  const TileItems it = tile_items(T);
+  // STRIDE: every gridDim-th rank from tile_rank0(); otherwise the grid covers
+  // the ranks and a workgroup takes one
+  for (uint32_t r = tile_rank0();; r += gridDim.x) {
+    uint32_t blk;
+    const int mode = tile_pick(T, it, r, blk);
+    if (mode == 0) break;
+    if (mode == 1) {
+      final_tile<BITS, VORT, XSPH, TeamBlock, TileCap<BITS>::value>(P, blk, L, use_list, flags, Q, QX, VS, OM, keys, cs,
+                                                                    ce, CS, ncnt, Xo, Po, Vo, Co, Fo, lds.big.sh,
+                                                                    lds.big.q, lds.big.gi);
+      if (!STRIDE) break;
+      __syncthreads();  // the next rank may be small tiles in the same LDS bytes
+    } else {
+      auto& reg = lds.small[threadIdx.x >> 6];
+      final_tile<BITS, VORT, XSPH, TeamWave, TileCap<BITS>::small>(P, blk, L, use_list, flags, Q, QX, VS, OM, keys, cs,
+                                                                   ce, CS, ncnt, Xo, Po, Vo, Co, Fo, reg.sh, reg.q, reg.gi);
+      if (!STRIDE) break;
+    }
   }
 }
 
@@ -895,97 +944,111 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_final(
 // Host launchers.
 static NeighbourList nlist(const StepBuffers& B) { return NeighbourList{B.NL, B.NLC, B.nl_cap, B.STATS}; }
 
-hipError_t launch_tile_lambda(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
-                              const StepBuffers& B, const float4* Q, const uint32_t* keys, bool first) {
+hipError_t launch_tile_lambda(hipStream_t st, int bits, uint32_t grid, bool stride, const DevParams& P,
+                              const TileList& T, const StepBuffers& B, const float4* Q, const uint32_t* keys,
+                              bool first) {
   dim3 g(grid), b(kTileThreads);
   const NeighbourList L = nlist(B);
+#define PBD_LAMBDA(BITS_, F_, S_)                                                                                \
+  hipLaunchKernelGGL((k_tile_lambda<BITS_, F_, S_>), g, b, 0, st, P, T, L, Q, keys, B.cstart, B.cend, B.LAM, B.DENS, \
+                     B.NCNT, B.CCNT)
+#define PBD_LAMBDA_S(BITS_, F_) \
+  if (stride) PBD_LAMBDA(BITS_, F_, true); \
+  else PBD_LAMBDA(BITS_, F_, false)
   if (bits == 5) {
-    if (first)
-      hipLaunchKernelGGL((k_tile_lambda<5, true>), g, b, 0, st, P, T, L, Q, keys, B.cstart, B.cend, B.LAM, B.DENS,
-                         B.NCNT, B.CCNT);
-    else
-      hipLaunchKernelGGL((k_tile_lambda<5, false>), g, b, 0, st, P, T, L, Q, keys, B.cstart, B.cend, B.LAM, B.DENS,
-                         B.NCNT, B.CCNT);
+    if (first) PBD_LAMBDA_S(5, true);
+    else PBD_LAMBDA_S(5, false);
   } else {
-    if (first)
-      hipLaunchKernelGGL((k_tile_lambda<6, true>), g, b, 0, st, P, T, L, Q, keys, B.cstart, B.cend, B.LAM, B.DENS,
-                         B.NCNT, B.CCNT);
-    else
-      hipLaunchKernelGGL((k_tile_lambda<6, false>), g, b, 0, st, P, T, L, Q, keys, B.cstart, B.cend, B.LAM, B.DENS,
-                         B.NCNT, B.CCNT);
+    if (first) PBD_LAMBDA_S(6, true);
+    else PBD_LAMBDA_S(6, false);
   }
+#undef PBD_LAMBDA_S
+#undef PBD_LAMBDA
   return hipGetLastError();
 }
 
 template <int BITS, bool COLLIDE, int NS>
-static void launch_delta_t(hipStream_t st, uint32_t grid, const DevParams& P, const TileList& T,
+static void launch_delta_t(hipStream_t st, uint32_t grid, bool stride, const DevParams& P, const TileList& T,
                            const StepBuffers& B, int use_list, uint32_t flags, const float4* Q, const float4* QX,
                            float4* Qo, float4* QXo, const uint32_t* keys) {
-  hipLaunchKernelGGL((k_tile_delta<BITS, COLLIDE, NS>), dim3(grid), dim3(kTileThreads), 0, st, P, T, nlist(B),
-                     use_list, flags, Q, QX, B.LAM, keys, B.cstart, B.cend, Qo, QXo, B.DP, B.VS);
+  if (stride)
+    hipLaunchKernelGGL((k_tile_delta<BITS, COLLIDE, NS, true>), dim3(grid), dim3(kTileThreads), 0, st, P, T, nlist(B),
+                       use_list, flags, Q, QX, B.LAM, keys, B.cstart, B.cend, Qo, QXo, B.DP, B.VS);
+  else
+    hipLaunchKernelGGL((k_tile_delta<BITS, COLLIDE, NS, false>), dim3(grid), dim3(kTileThreads), 0, st, P, T, nlist(B),
+                       use_list, flags, Q, QX, B.LAM, keys, B.cstart, B.cend, Qo, QXo, B.DP, B.VS);
 }
 
 template <int BITS>
-static void launch_delta_b(hipStream_t st, uint32_t grid, const DevParams& P, const TileList& T,
+static void launch_delta_b(hipStream_t st, uint32_t grid, bool stride, const DevParams& P, const TileList& T,
                            const StepBuffers& B, int use_list, uint32_t flags, bool collide, const float4* Q,
                            const float4* QX, float4* Qo, float4* QXo, const uint32_t* keys) {
   const bool n4 = P.n_scorr == 4;  // the s_corr exponent of Macklin & Mueller 2013
   if (collide) {
-    if (n4) launch_delta_t<BITS, true, 4>(st, grid, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
-    else launch_delta_t<BITS, true, 0>(st, grid, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
+    if (n4) launch_delta_t<BITS, true, 4>(st, grid, stride, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
+    else launch_delta_t<BITS, true, 0>(st, grid, stride, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
   } else {
-    if (n4) launch_delta_t<BITS, false, 4>(st, grid, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
-    else launch_delta_t<BITS, false, 0>(st, grid, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
+    if (n4) launch_delta_t<BITS, false, 4>(st, grid, stride, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
+    else launch_delta_t<BITS, false, 0>(st, grid, stride, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
   }
 }
 
-hipError_t launch_tile_delta(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
-                             const StepBuffers& B, uint32_t flags, bool collide, bool use_list, const float4* Q,
-                             const float4* QX, float4* Qo, float4* QXo, const uint32_t* keys) {
+hipError_t launch_tile_delta(hipStream_t st, int bits, uint32_t grid, bool stride, const DevParams& P,
+                             const TileList& T, const StepBuffers& B, uint32_t flags, bool collide, bool use_list,
+                             const float4* Q, const float4* QX, float4* Qo, float4* QXo, const uint32_t* keys) {
   const int ul = use_list ? 1 : 0;
-  if (bits == 5) launch_delta_b<5>(st, grid, P, T, B, ul, flags, collide, Q, QX, Qo, QXo, keys);
-  else launch_delta_b<6>(st, grid, P, T, B, ul, flags, collide, Q, QX, Qo, QXo, keys);
+  if (bits == 5) launch_delta_b<5>(st, grid, stride, P, T, B, ul, flags, collide, Q, QX, Qo, QXo, keys);
+  else launch_delta_b<6>(st, grid, stride, P, T, B, ul, flags, collide, Q, QX, Qo, QXo, keys);
   return hipGetLastError();
 }
 
-hipError_t launch_tile_omega(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
-                             const StepBuffers& B, const float4* Q, const uint32_t* keys, bool xsph) {
+hipError_t launch_tile_omega(hipStream_t st, int bits, uint32_t grid, bool stride, const DevParams& P,
+                             const TileList& T, const StepBuffers& B, const float4* Q, const uint32_t* keys,
+                             bool xsph) {
   dim3 g(grid), b(kTileThreads);
   const NeighbourList L = nlist(B);
-#define PBD_OMEGA(BITS_, X_) \
-  hipLaunchKernelGGL((k_tile_omega<BITS_, X_>), g, b, 0, st, P, T, L, Q, B.VS, keys, B.cstart, B.cend, B.OM, B.V)
+#define PBD_OMEGA(BITS_, X_, S_) \
+  hipLaunchKernelGGL((k_tile_omega<BITS_, X_, S_>), g, b, 0, st, P, T, L, Q, B.VS, keys, B.cstart, B.cend, B.OM, B.V)
+#define PBD_OMEGA_S(BITS_, X_) \
+  if (stride) PBD_OMEGA(BITS_, X_, true); \
+  else PBD_OMEGA(BITS_, X_, false)
   if (bits == 5) {
-    if (xsph) PBD_OMEGA(5, true);
-    else PBD_OMEGA(5, false);
+    if (xsph) PBD_OMEGA_S(5, true);
+    else PBD_OMEGA_S(5, false);
   } else {
-    if (xsph) PBD_OMEGA(6, true);
-    else PBD_OMEGA(6, false);
+    if (xsph) PBD_OMEGA_S(6, true);
+    else PBD_OMEGA_S(6, false);
   }
+#undef PBD_OMEGA_S
 #undef PBD_OMEGA
   return hipGetLastError();
 }
 
-hipError_t launch_tile_final(hipStream_t st, int bits, uint32_t grid, const DevParams& P, const TileList& T,
-                             const StepBuffers& B, uint32_t flags, bool use_list, const float4* Q, const float4* QX,
-                             const uint32_t* keys) {
+hipError_t launch_tile_final(hipStream_t st, int bits, uint32_t grid, bool stride, const DevParams& P,
+                             const TileList& T, const StepBuffers& B, uint32_t flags, bool use_list, const float4* Q,
+                             const float4* QX, const uint32_t* keys) {
   dim3 g(grid), b(kTileThreads);
   const NeighbourList L = nlist(B);
   const int ul = use_list ? 1 : 0;
   const bool vort = (flags & FF_VORT) != 0, xsph = (flags & FF_XSPH) != 0;
-#define PBD_FINAL(BITS_, V_, X_)                                                                                \
-  hipLaunchKernelGGL((k_tile_final<BITS_, V_, X_>), g, b, 0, st, P, T, L, ul, flags, Q, QX, B.VS, B.OM, keys,   \
+#define PBD_FINAL(BITS_, V_, X_, S_)                                                                             \
+  hipLaunchKernelGGL((k_tile_final<BITS_, V_, X_, S_>), g, b, 0, st, P, T, L, ul, flags, Q, QX, B.VS, B.OM, keys, \
                      B.cstart, B.cend, B.CS, B.NCNT, B.X, B.P, B.V, B.C, B.F)
+#define PBD_FINAL_S(BITS_, V_, X_) \
+  if (stride) PBD_FINAL(BITS_, V_, X_, true); \
+  else PBD_FINAL(BITS_, V_, X_, false)
   if (bits == 5) {
-    if (vort && xsph) PBD_FINAL(5, true, true);
-    else if (vort) PBD_FINAL(5, true, false);
-    else if (xsph) PBD_FINAL(5, false, true);
-    else PBD_FINAL(5, false, false);
+    if (vort && xsph) PBD_FINAL_S(5, true, true);
+    else if (vort) PBD_FINAL_S(5, true, false);
+    else if (xsph) PBD_FINAL_S(5, false, true);
+    else PBD_FINAL_S(5, false, false);
   } else {
-    if (vort && xsph) PBD_FINAL(6, true, true);
-    else if (vort) PBD_FINAL(6, true, false);
-    else if (xsph) PBD_FINAL(6, false, true);
-    else PBD_FINAL(6, false, false);
+    if (vort && xsph) PBD_FINAL_S(6, true, true);
+    else if (vort) PBD_FINAL_S(6, true, false);
+    else if (xsph) PBD_FINAL_S(6, false, true);
+    else PBD_FINAL_S(6, false, false);
   }
+#undef PBD_FINAL_S
 #undef PBD_FINAL
   return hipGetLastError();
 }

@@ -308,6 +308,49 @@ def test_gpu_batch_matches_single_scenes(F, O):
     b.close()
 
 
+def test_gpu_batch_strided_tile_grid(F, O):
+    """256 scenes of 10x10x11 particles (281,600 in all) in 3.2-wide enclosures:
+    each scene takes a 64-cell Morton slot, so the cell table bounds 2^21 tiles and
+    the tile grid min(tiles, particles) = 281,600 exceeds the 2^18 workgroups a
+    direct grid may hold.  The neighbour passes then run on the capped grid, each
+    workgroup taking every gridDim-th work item.  Three scenes step as the oracle
+    steps them alone; every scene keeps sorted keys and finite state."""
+    scenes = []
+    for k in range(256):
+        p = small_dam(F, 10)
+        p.box_depth = 11
+        p.bounds_max[0] = p.bounds_max[1] = p.bounds_max[2] = 3.2
+        p.relaxation = 300.0 + 2.0 * k
+        p.box_seed_lo = 100 + k
+        scenes.append(p)
+    b = F.capi.Batch(scenes)
+    total = sum(b.scene(s)[0] for s in range(256))
+    assert total == 281600
+    flush(f"[batch strided] {len(scenes)} scenes, {total} particles")
+    b.compute()
+    first = 0
+    for s in range(256):
+        cnt, off, wp = b.scene(s)
+        keys = b.download(s, "cell_ids")
+        assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+        x = b.download(s, "positions", frame=b.FRAME_BATCH)
+        assert np.isfinite(x).all()
+        if s in (0, 128, 255):
+            pos, vel, col = F.generate_box(wp)
+            out = O.step(wp, O.State(pos, vel, col))
+            h = O.derive(wp).h_spiky
+            assert np.array_equal(keys, out["keys"])
+            assert np.array_equal(b.download(s, "particle_ids") - first, out["ids"])
+            ex = np.abs(x[:, :3] - out["x4"][:, :3]).max()
+            # positions are stored in the batch frame: a far slot's coordinates (up to ~50
+            # here) are resolved to one FP32 ulp, which exceeds 1e-5 h above |x| = 32
+            ulp = float(np.spacing(np.float32(np.abs(out["x4"][:, :3]).max())))
+            flush(f"[batch strided] scene {s}: max|dx|={ex:.3g} (ulp {ulp:.3g})")
+            assert ex <= max(1e-5 * h, ulp)
+        first += cnt
+    b.close()
+
+
 def test_gpu_batch_graph_and_errors(F, O):
     scenes = _batch_scenes(F, 3)
     a, g = F.capi.Batch(scenes), F.capi.Batch(scenes)
