@@ -336,7 +336,7 @@ int Fluid::enqueue_substep() {
   const uint32_t g = P.gates;
   // LDS tiles for the 27-cell stencil of the textbook kernels (tiles.hip)
   const int bits = (int)params_.tile_bits;
-  const bool tiles = bits != 0 && P.variant == PBD_KERNEL_TEXTBOOK && P.kw == 1 && (g & G_FIND_CONTACTS) && !P.phased;
+  bool tiles = bits != 0 && P.variant == PBD_KERNEL_TEXTBOOK && P.kw == 1 && (g & G_FIND_CONTACTS);
   uint32_t tgrid = 0, ntiles = 0;
   bool tstride = false;
   if (tiles) {
@@ -346,6 +346,8 @@ int Fluid::enqueue_substep() {
     tgrid = (std::min(ntiles, P.n) + kTileGridMultiple - 1u) / kTileGridMultiple * kTileGridMultiple;
     tstride = tgrid > kTileGridDirect;
     if (tstride) tgrid = kTileGridStride;
+    // phases on tiles: 2^5-cell blocks with the direct grid only
+    if (P.phased && (bits != 5 || tstride)) tiles = false;
   }
   const TileList TL{tile_list_, tile_list_ + cap_, tile_count_};
   const size_t n16 = (size_t)P.n * 16, n4 = (size_t)P.n * 4;

@@ -63,10 +63,11 @@ constexpr uint32_t kTileCompactChunk = 4096;  // flags per block of the compacti
 constexpr uint32_t kTileChunk = PBD_TILE_CHUNK;        // consecutive tiles per XCD turn (tiles.hip tile_rank0)
 constexpr uint32_t kTileGridMultiple = 8 * kTileChunk;  // tile grids are a multiple of (XCDs x tile chunk)
 // A tile grid covers the work items (one per workgroup) up to kTileGridDirect
-// workgroups.  Above that (a batch's cell table bounds the items far above the
-// occupied blocks) the passes run on kTileGridStride workgroups, each taking
-// every gridDim-th item.
-constexpr uint32_t kTileGridDirect = 1u << 18;
+// workgroups (a 256x128x128-cell grid bounds 327,680).  Above that (a batch's
+// cell table bounds the items far above the occupied blocks: ~2M workgroups for
+// 256 scenes) the passes run on kTileGridStride workgroups, each taking every
+// gridDim-th item.
+constexpr uint32_t kTileGridDirect = 1u << 20;
 constexpr uint32_t kTileGridStride = 16384;
 static_assert(kTileGridStride % kTileGridMultiple == 0 && kTileGridDirect % kTileGridMultiple == 0, "tile grids");
 // Splits the occupied blocks (start[b] != 0) into large[] and small[] by their

@@ -403,17 +403,22 @@ __device__ __forceinline__ void omega_particle(const DevParams& P, uint32_t s, c
   OM[s] = omega_finish<VAR>(P, a);
 }
 
-// XSPH alone for particle s, with final_particle's neighbour order and arithmetic.
+// XSPH alone for particle s, with final_particle's neighbour order, arithmetic
+// and phase masks (P.phs: a solid particle gets none, a fluid one sums over
+// fluid neighbours).
 template <int VAR>
 __device__ __forceinline__ float4 xsph_particle(const DevParams& P, uint32_t s, const float4* __restrict__ Q,
                                                 const float4* __restrict__ VS, const uint32_t* __restrict__ keys,
                                                 const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce) {
+  const uint32_t* __restrict__ ph = P.phs;
   const float4 qi = Q[s], vi = VS[s];
+  if (ph && ph[s]) return vi;
   const CellFrame f = make_frame(P, keys[s]);
   FinalAcc a;
   uint32_t nc = 0;
   visit_neighbours(P, s, f, qi.x, qi.y, qi.z, (P.gates & G_FIND_CONTACTS) != 0, (P.gates & G_FIND_NEIGHBOURS) != 0, cs,
                    ce, nc, [&](uint32_t j, float bx, float by, float bz, bool) {
+                     if (ph && ph[j]) return;
                      const float4 qj = Q[j];
                      final_pair_textbook<VAR>(P, false, true, bx - qj.x, by - qj.y, bz - qj.z, vi, VS[j], 0.0f, 0.0f,
                                               a);

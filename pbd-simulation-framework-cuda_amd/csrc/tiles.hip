@@ -174,6 +174,13 @@ __device__ __forceinline__ void tile_locate(const TileShared<BITS>& sh, uint32_t
   slot = sh.lstart[h] + (g - sh.gstart[h]);
 }
 
+// Sorted index of the particle in LDS slot j.
+template <int BITS>
+__device__ __forceinline__ uint32_t tile_global(const TileShared<BITS>& sh, uint32_t j) {
+  const int c = tile_cell_of<BITS>(sh, j);
+  return sh.gstart[c] + (j - sh.lstart[c]);
+}
+
 // Walk the 27 stencil cells of halo cell h; body(j) for every LDS slot j != slot.
 template <int BITS, class Body>
 __device__ __forceinline__ uint32_t tile_stencil(const TileShared<BITS>& sh, int h, uint32_t slot, Body body) {
@@ -535,12 +542,19 @@ struct ChunkWriter {
   }
 };
 
+// A list entry is an LDS slot (< 2^15); with phases, bit 15 marks a solid
+// neighbour (the producing pass holds the solid flag in the slot's .w).
+constexpr uint32_t kNlSolid = 0x8000u;
+
+template <bool PH>
 __device__ __forceinline__ void nl_push_bits(ChunkWriter& cw, const NeighbourList& L, uint32_t g, uint32_t base,
-                                             uint32_t m) {
+                                             uint32_t m, const float4* s_q) {
   while (m) {
     const uint32_t k = __builtin_ctz(m);
     m &= m - 1u;
-    cw.push(L, g, base + k);
+    uint32_t e = base + k;
+    if (PH && s_q[e].w != 0.0f) e |= kNlSolid;
+    cw.push(L, g, e);
   }
 }
 
@@ -548,7 +562,7 @@ __device__ __forceinline__ void nl_push_bits(ChunkWriter& cw, const NeighbourLis
 // padded to a multiple of 8 with the sentinel slot CAP.  Returns the padded
 // length, or kNlInvalid when a stencil column holds more than 64 particles or
 // more than kNlMax pairs are in range (the list is not written then).
-template <int BITS, int CAP>
+template <int BITS, int CAP, bool PH>
 __device__ __forceinline__ uint32_t nl_build(const DevParams& P, const NeighbourList& L, uint32_t g,
                                              const TileShared<BITS>& sh, const float4* s_q, int h, uint32_t slot,
                                              float4 qi) {
@@ -574,8 +588,8 @@ __device__ __forceinline__ uint32_t nl_build(const DevParams& P, const Neighbour
   ChunkWriter cw;
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
-    nl_push_bits(cw, L, g, base[c], lo[c]);
-    nl_push_bits(cw, L, g, base[c] + 32u, hi[c]);
+    nl_push_bits<PH>(cw, L, g, base[c], lo[c], s_q);
+    nl_push_bits<PH>(cw, L, g, base[c] + 32u, hi[c], s_q);
   }
   while (cw.k != 0u) cw.push(L, g, (uint32_t)CAP);
   return cw.c * 8u;
@@ -583,7 +597,7 @@ __device__ __forceinline__ uint32_t nl_build(const DevParams& P, const Neighbour
 
 // lambda over one tile (team: workgroup or wave, CAP: its LDS capacity),
 // building the neighbour lists.
-template <int BITS, bool FIRST, class Team, int CAP>
+template <int BITS, bool FIRST, bool PH, class Team, int CAP>
 __device__ __forceinline__ void lambda_tile(const DevParams& P, uint32_t blk, const NeighbourList& L,
                                             const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
                                             const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
@@ -603,7 +617,9 @@ __device__ __forceinline__ void lambda_tile(const DevParams& P, uint32_t blk, co
     if (tid == 0) atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_BLOCK], (unsigned long long)(g1 - g0));
     return;
   }
-  tile_load<BITS, CAP, Team>(P, sh, total, Q, ix0, iy0, iz0, s_q, nullptr, 0u, [](uint32_t) { return 0.0f; });
+  // .w: 1 for a solid particle (PH), else 0
+  tile_load<BITS, CAP, Team>(P, sh, total, Q, ix0, iy0, iz0, s_q, nullptr, 0u,
+                             [&](uint32_t gi) { return PH && P.phs[gi] ? 1.0f : 0.0f; });
   Team::sync();
   for (uint32_t g = g0 + tid; g < g1; g += Team::kSize) {
     int h;
@@ -619,10 +635,12 @@ __device__ __forceinline__ void lambda_tile(const DevParams& P, uint32_t blk, co
     const float4 qi = s_q[slot];
     LambdaAcc a;
     const auto pair = [&](uint32_t j) __attribute__((always_inline)) {
+      if (PH && (j & kNlSolid)) return;  // a fluid particle counts fluid neighbours only
       const float4 qj = s_q[j];
+      if (PH && qj.w != 0.0f) return;
       lambda_pair<VAR>(P, qi.x - qj.x, qi.y - qj.y, qi.z - qj.z, a);
     };
-    const uint32_t n = nl_build<BITS, CAP>(P, L, g, sh, s_q, h, slot, qi);
+    const uint32_t n = nl_build<BITS, CAP, PH>(P, L, g, sh, s_q, h, slot, qi);
     L.cnt[g] = (uint16_t)n;
     if (n != kNlInvalid) {
       nl_visit(L, g, n, pair);
@@ -631,7 +649,8 @@ __device__ __forceinline__ void lambda_tile(const DevParams& P, uint32_t blk, co
       atomicAdd(&L.stats[PBD_COUNTER_UNLISTED_LONG], 1ull);
     }
     float W;
-    lam[g] = lambda_finish<VAR>(P, a, W, scene_eps(P, keys[g]));
+    const float l = lambda_finish<VAR>(P, a, W, scene_eps(P, keys[g]));
+    lam[g] = PH && qi.w != 0.0f ? 0.0f : l;  // a solid particle has no density constraint
     if (FIRST) {
       dens[g] = W;
       ncnt[g] = cnt;
@@ -640,7 +659,7 @@ __device__ __forceinline__ void lambda_tile(const DevParams& P, uint32_t blk, co
   }
 }
 
-template <int BITS, bool FIRST, bool STRIDE>
+template <int BITS, bool FIRST, bool STRIDE, bool PH>
 __global__ __launch_bounds__(kTileThreads) void k_tile_lambda(DevParams P, TileList T, NeighbourList L,
                                                               const float4* __restrict__ Q,
                                                               const uint32_t* __restrict__ keys,
@@ -658,13 +677,13 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_lambda(DevParams P, TileL
     const int mode = tile_pick(T, it, r, blk);
     if (mode == 0) break;
     if (mode == 1) {
-      lambda_tile<BITS, FIRST, TeamBlock, TileCap<BITS>::value>(P, blk, L, Q, keys, cs, ce, lam, dens, ncnt, ccnt,
+      lambda_tile<BITS, FIRST, PH, TeamBlock, TileCap<BITS>::value>(P, blk, L, Q, keys, cs, ce, lam, dens, ncnt, ccnt,
                                                                 lds.big.sh, lds.big.q);
       if (!STRIDE) break;
       __syncthreads();  // the next rank may be small tiles in the same LDS bytes
     } else {
       auto& reg = lds.small[threadIdx.x >> 6];
-      lambda_tile<BITS, FIRST, TeamWave, TileCap<BITS>::small>(P, blk, L, Q, keys, cs, ce, lam, dens, ncnt, ccnt, reg.sh,
+      lambda_tile<BITS, FIRST, PH, TeamWave, TileCap<BITS>::small>(P, blk, L, Q, keys, cs, ce, lam, dens, ncnt, ccnt, reg.sh,
                                                                reg.q);
       if (!STRIDE) break;
     }
@@ -673,7 +692,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_lambda(DevParams P, TileL
 
 // Delta-p (+ collisions, apply, update) over one tile, over the lambda pass's
 // lists (use_list) or a scan of the stencil.
-template <int BITS, bool COLLIDE, int NS, class Team, int CAP>
+template <int BITS, bool COLLIDE, int NS, bool PH, class Team, int CAP>
 __device__ __forceinline__ void delta_tile(const DevParams& P, uint32_t blk, const NeighbourList& L, int use_list,
                                            uint32_t flags, const float4* __restrict__ Q,
                                            const float4* __restrict__ QX, const float* __restrict__ lam,
@@ -699,15 +718,45 @@ __device__ __forceinline__ void delta_tile(const DevParams& P, uint32_t blk, con
     uint32_t slot;
     tile_locate<BITS>(sh, key, g, ix0, iy0, iz0, h, slot);
     const uint32_t n = use_list ? (uint32_t)L.cnt[g] : (uint32_t)kNlInvalid;
-    if (n == kNlInvalid && tile_count<BITS>(sh, h) > P.cap) {
+    // with phases, a particle without a list takes the global path (the tile holds
+    // no phases for a scan)
+    if (n == kNlInvalid && (PH || tile_count<BITS>(sh, h) > P.cap)) {
       delta_particle<VAR, true, COLLIDE>(P, flags, g, Q, QX, lam, keys, cs, ce, Qo, QXo, DP, VS);
       continue;
     }
     const float4 qb = s_q[slot];
+    // phases: delta p between fluid particles only; the contact push on every pair,
+    // friction on contacts with a solid (displacements p* - x from the global arrays)
+    const bool si = PH && P.phs[g] != 0u;
+    const bool fric = PH && COLLIDE && P.mu_s > 0.0f;
+    float dix = 0.0f, diy = 0.0f, diz = 0.0f;
+    if (fric) {
+      const float4 qg = Q[g], xg = QX[g];
+      dix = qg.x - xg.x;
+      diy = qg.y - xg.y;
+      diz = qg.z - xg.z;
+    }
     DeltaAcc a;
-    const auto pair = [&](uint32_t j) __attribute__((always_inline)) {
+    const auto pair = [&](uint32_t e) __attribute__((always_inline)) {
+      const uint32_t j = PH ? (e & ~kNlSolid) : e;
       const float4 qj = s_q[j];
-      delta_pair<VAR, true, COLLIDE, NS>(P, qb.w, qj.w, qb.x - qj.x, qb.y - qj.y, qb.z - qj.z, true, a);
+      const float rx = qb.x - qj.x, ry = qb.y - qj.y, rz = qb.z - qj.z;
+      if (PH && (si || (e & kNlSolid))) {
+        delta_pair<VAR, false, COLLIDE, NS>(P, 0.0f, 0.0f, rx, ry, rz, true, a);
+        if (fric) {
+          const float r2 = rx * rx + ry * ry + rz * rz;
+          if (r2 > 0.0f) {
+            const float il = rsq(r2), len = r2 * il;
+            if (len < P.diameter && len > 0.00001f) {
+              const uint32_t gj = tile_global<BITS>(sh, j);
+              const float4 qgj = Q[gj], xj = QX[gj];
+              friction_pair(P, rx, ry, rz, len, il, dix, diy, diz, qgj.x - xj.x, qgj.y - xj.y, qgj.z - xj.z, a);
+            }
+          }
+        }
+      } else {
+        delta_pair<VAR, true, COLLIDE, NS>(P, qb.w, qj.w, rx, ry, rz, true, a);
+      }
     };
     if (n != kNlInvalid) nl_visit(L, g, n, pair);
     else tile_stencil<BITS>(sh, h, slot, pair);
@@ -718,7 +767,7 @@ __device__ __forceinline__ void delta_tile(const DevParams& P, uint32_t blk, con
   }
 }
 
-template <int BITS, bool COLLIDE, int NS, bool STRIDE>
+template <int BITS, bool COLLIDE, int NS, bool STRIDE, bool PH>
 __global__ __launch_bounds__(kTileThreads) void k_tile_delta(DevParams P, TileList T, NeighbourList L, int use_list,
                                                              uint32_t flags, const float4* __restrict__ Q,
                                                              const float4* __restrict__ QX,
@@ -737,13 +786,13 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_delta(DevParams P, TileLi
     const int mode = tile_pick(T, it, r, blk);
     if (mode == 0) break;
     if (mode == 1) {
-      delta_tile<BITS, COLLIDE, NS, TeamBlock, TileCap<BITS>::value>(P, blk, L, use_list, flags, Q, QX, lam, keys, cs,
+      delta_tile<BITS, COLLIDE, NS, PH, TeamBlock, TileCap<BITS>::value>(P, blk, L, use_list, flags, Q, QX, lam, keys, cs,
                                                                      ce, Qo, QXo, DP, VS, lds.big.sh, lds.big.q);
       if (!STRIDE) break;
       __syncthreads();  // the next rank may be small tiles in the same LDS bytes
     } else {
       auto& reg = lds.small[threadIdx.x >> 6];
-      delta_tile<BITS, COLLIDE, NS, TeamWave, TileCap<BITS>::small>(P, blk, L, use_list, flags, Q, QX, lam, keys, cs,
+      delta_tile<BITS, COLLIDE, NS, PH, TeamWave, TileCap<BITS>::small>(P, blk, L, use_list, flags, Q, QX, lam, keys, cs,
                                                                     ce, Qo, QXo, DP, VS, reg.sh, reg.q);
       if (!STRIDE) break;
     }
@@ -753,7 +802,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_delta(DevParams P, TileLi
 // XSPH: the same neighbour walk also applies XSPH (computeViscosity,
 // Density.h:280-310) and writes the velocities to Vo, in the order and with
 // the arithmetic of the final pass, which then leaves them alone.
-template <int BITS, bool XSPH, class Team, int CAP>
+template <int BITS, bool XSPH, bool PH, class Team, int CAP>
 __device__ __forceinline__ void omega_tile(const DevParams& P, uint32_t blk, const NeighbourList& L,
                                            const float4* __restrict__ Q, const float4* __restrict__ VS,
                                            const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
@@ -775,7 +824,9 @@ __device__ __forceinline__ void omega_tile(const DevParams& P, uint32_t blk, con
     return;
   }
   // global index of each slot: velocities are gathered from L2
-  tile_load<BITS, CAP, Team>(P, sh, total, Q, ix0, iy0, iz0, s_q, s_gi, g0, [](uint32_t) { return 0.0f; });
+  // .w: 1 for a solid particle (PH), else 0
+  tile_load<BITS, CAP, Team>(P, sh, total, Q, ix0, iy0, iz0, s_q, s_gi, g0,
+                             [&](uint32_t gi) { return PH && P.phs[gi] ? 1.0f : 0.0f; });
   Team::sync();
   for (uint32_t g = g0 + tid; g < g1; g += Team::kSize) {
     int h;
@@ -795,13 +846,17 @@ __device__ __forceinline__ void omega_tile(const DevParams& P, uint32_t blk, con
     const float vx = v4.x, vy = v4.y, vz = v4.z, vw = v4.w;
     OmegaAcc a;
     FinalAcc xa;
+    const bool si = PH && qi.w != 0.0f;  // a solid particle: omega 0, no XSPH
     const auto pair = [&](uint32_t j) __attribute__((always_inline)) {
-      const float4 qj = s_q[j], vj = VS[s_gi[j]];
+      if (PH && (si || (j & kNlSolid))) return;  // fluid neighbours only
+      const float4 qj = s_q[j];
+      if (PH && qj.w != 0.0f) return;
+      const float4 vj = VS[s_gi[j]];
       const float rx = qi.x - qj.x, ry = qi.y - qj.y, rz = qi.z - qj.z;
       omega_pair<VAR>(P, rx, ry, rz, vj.x - vx, vj.y - vy, vj.z - vz, a);
       if (XSPH) final_pair_textbook<VAR>(P, false, true, rx, ry, rz, make_float4(vx, vy, vz, vw), vj, 0.0f, 0.0f, xa);
     };
-    const uint32_t n = nl_build<BITS, CAP>(P, L, g, sh, s_q, h, slot, qi);
+    const uint32_t n = nl_build<BITS, CAP, PH>(P, L, g, sh, s_q, h, slot, qi);
     L.cnt[g] = (uint16_t)n;
     if (n != kNlInvalid) {
       nl_visit(L, g, n, pair);
@@ -820,7 +875,7 @@ __device__ __forceinline__ void omega_tile(const DevParams& P, uint32_t blk, con
 // XSPH: the same neighbour walk also applies XSPH (computeViscosity,
 // Density.h:280-310) and writes the velocities to Vo, in the order and with
 // the arithmetic of the final pass, which then leaves them alone.
-template <int BITS, bool XSPH, bool STRIDE>
+template <int BITS, bool XSPH, bool STRIDE, bool PH>
 __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileList T, NeighbourList L,
                                                              const float4* __restrict__ Q,
                                                              const float4* __restrict__ VS,
@@ -838,19 +893,19 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_omega(DevParams P, TileLi
     const int mode = tile_pick(T, it, r, blk);
     if (mode == 0) break;
     if (mode == 1) {
-      omega_tile<BITS, XSPH, TeamBlock, TileCap<BITS>::value>(P, blk, L, Q, VS, keys, cs, ce, OM, Vo, lds.big.sh,
+      omega_tile<BITS, XSPH, PH, TeamBlock, TileCap<BITS>::value>(P, blk, L, Q, VS, keys, cs, ce, OM, Vo, lds.big.sh,
                                                               lds.big.q, lds.big.gi);
       if (!STRIDE) break;
       __syncthreads();  // the next rank may be small tiles in the same LDS bytes
     } else {
       auto& reg = lds.small[threadIdx.x >> 6];
-      omega_tile<BITS, XSPH, TeamWave, TileCap<BITS>::small>(P, blk, L, Q, VS, keys, cs, ce, OM, Vo, reg.sh, reg.q, reg.gi);
+      omega_tile<BITS, XSPH, PH, TeamWave, TileCap<BITS>::small>(P, blk, L, Q, VS, keys, cs, ce, OM, Vo, reg.sh, reg.q, reg.gi);
       if (!STRIDE) break;
     }
   }
 }
 
-template <int BITS, bool VORT, bool XSPH, class Team, int CAP>
+template <int BITS, bool VORT, bool XSPH, bool PH, class Team, int CAP>
 __device__ __forceinline__ void final_tile(const DevParams& P, uint32_t blk, const NeighbourList& L, int use_list,
                                            uint32_t flags, const float4* __restrict__ Q,
                                            const float4* __restrict__ QX, const float4* __restrict__ VS,
@@ -867,8 +922,10 @@ __device__ __forceinline__ void final_tile(const DevParams& P, uint32_t blk, con
   const uint32_t g0 = sh.g0, g1 = sh.g1;
   constexpr bool vort = VORT, xsph = XSPH;  // flags & FF_VORT, flags & FF_XSPH
   if (total > (uint32_t)CAP) {
-    for (uint32_t g = g0 + tid; g < g1; g += Team::kSize)
+    for (uint32_t g = g0 + tid; g < g1; g += Team::kSize) {
+      if (PH) P.ph[g] = P.phs[g];  // phases follow the particles into the sorted order
       final_particle<VAR>(P, flags, g, Q, QX, VS, OM, keys, cs, ce, CS, ncnt, Xo, Po, Vo, Co, Fo);
+    }
     return;
   }
   // global index of each slot (XSPH only), for the velocity gathers from L2
@@ -876,15 +933,19 @@ __device__ __forceinline__ void final_tile(const DevParams& P, uint32_t blk, con
   tile_load<BITS, CAP, Team>(P, sh, total, Q, ix0, iy0, iz0, s_q, XSPH ? s_gi : nullptr, g0, omega_len);
   Team::sync();
   for (uint32_t g = g0 + tid; g < g1; g += Team::kSize) {
+    if (PH) P.ph[g] = P.phs[g];  // phases follow the particles into the sorted order
     const uint32_t key = keys[g];
     int h;
     uint32_t slot;
     tile_locate<BITS>(sh, key, g, ix0, iy0, iz0, h, slot);
     const uint32_t n = use_list ? (uint32_t)L.cnt[g] : (uint32_t)kNlInvalid;
-    if (n == kNlInvalid && tile_count<BITS>(sh, h) > P.cap) {
+    // with phases, a particle without a list takes the global path (the tile holds
+    // no phases for a scan)
+    if (n == kNlInvalid && (PH || tile_count<BITS>(sh, h) > P.cap)) {
       final_particle<VAR>(P, flags, g, Q, QX, VS, OM, keys, cs, ce, CS, ncnt, Xo, Po, Vo, Co, Fo);
       continue;
     }
+    const bool si = PH && P.phs[g] != 0u;  // a solid particle: no vorticity force, no XSPH
     const float4 qb = s_q[slot];
     // velocity components as scalars: copies of the float4 aggregate inside the
     // pair loop are not promoted to registers and end up in scratch
@@ -894,6 +955,7 @@ __device__ __forceinline__ void final_tile(const DevParams& P, uint32_t blk, con
     FinalAcc a;
     if (vort || xsph) {
       const auto pair = [&](uint32_t j) __attribute__((always_inline)) {
+        if (PH && (si || (j & kNlSolid))) return;  // fluid neighbours only
         const float4 qj = s_q[j];
         const float4 vj = XSPH ? VS[s_gi[j]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // unused without XSPH
         final_pair_textbook<VAR>(P, vort, xsph, qb.x - qj.x, qb.y - qj.y, qb.z - qj.z, make_float4(vx, vy, vz, vw),
@@ -910,7 +972,7 @@ __device__ __forceinline__ void final_tile(const DevParams& P, uint32_t blk, con
   }
 }
 
-template <int BITS, bool VORT, bool XSPH, bool STRIDE>
+template <int BITS, bool VORT, bool XSPH, bool STRIDE, bool PH>
 __global__ __launch_bounds__(kTileThreads) void k_tile_final(
     DevParams P, TileList T, NeighbourList L, int use_list, uint32_t flags, const float4* __restrict__ Q,
     const float4* __restrict__ QX, const float4* __restrict__ VS, const float4* __restrict__ OM,
@@ -926,14 +988,14 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_final(
     const int mode = tile_pick(T, it, r, blk);
     if (mode == 0) break;
     if (mode == 1) {
-      final_tile<BITS, VORT, XSPH, TeamBlock, TileCap<BITS>::value>(P, blk, L, use_list, flags, Q, QX, VS, OM, keys, cs,
+      final_tile<BITS, VORT, XSPH, PH, TeamBlock, TileCap<BITS>::value>(P, blk, L, use_list, flags, Q, QX, VS, OM, keys, cs,
                                                                     ce, CS, ncnt, Xo, Po, Vo, Co, Fo, lds.big.sh,
                                                                     lds.big.q, lds.big.gi);
       if (!STRIDE) break;
       __syncthreads();  // the next rank may be small tiles in the same LDS bytes
     } else {
       auto& reg = lds.small[threadIdx.x >> 6];
-      final_tile<BITS, VORT, XSPH, TeamWave, TileCap<BITS>::small>(P, blk, L, use_list, flags, Q, QX, VS, OM, keys, cs,
+      final_tile<BITS, VORT, XSPH, PH, TeamWave, TileCap<BITS>::small>(P, blk, L, use_list, flags, Q, QX, VS, OM, keys, cs,
                                                                    ce, CS, ncnt, Xo, Po, Vo, Co, Fo, reg.sh, reg.q, reg.gi);
       if (!STRIDE) break;
     }
@@ -944,18 +1006,24 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_final(
 // Host launchers.
 static NeighbourList nlist(const StepBuffers& B) { return NeighbourList{B.NL, B.NLC, B.nl_cap, B.STATS}; }
 
+// Phases (P.phased) are supported on 2^5-cell tiles with the direct grid; the
+// host keeps other phased configurations on the per-particle kernels.
 hipError_t launch_tile_lambda(hipStream_t st, int bits, uint32_t grid, bool stride, const DevParams& P,
                               const TileList& T, const StepBuffers& B, const float4* Q, const uint32_t* keys,
                               bool first) {
   dim3 g(grid), b(kTileThreads);
   const NeighbourList L = nlist(B);
-#define PBD_LAMBDA(BITS_, F_, S_)                                                                                \
-  hipLaunchKernelGGL((k_tile_lambda<BITS_, F_, S_>), g, b, 0, st, P, T, L, Q, keys, B.cstart, B.cend, B.LAM, B.DENS, \
-                     B.NCNT, B.CCNT)
+#define PBD_LAMBDA(BITS_, F_, S_, PH_)                                                                           \
+  hipLaunchKernelGGL((k_tile_lambda<BITS_, F_, S_, PH_>), g, b, 0, st, P, T, L, Q, keys, B.cstart, B.cend, B.LAM, \
+                     B.DENS, B.NCNT, B.CCNT)
 #define PBD_LAMBDA_S(BITS_, F_) \
-  if (stride) PBD_LAMBDA(BITS_, F_, true); \
-  else PBD_LAMBDA(BITS_, F_, false)
-  if (bits == 5) {
+  if (stride) PBD_LAMBDA(BITS_, F_, true, false); \
+  else PBD_LAMBDA(BITS_, F_, false, false)
+  if (P.phased) {
+    if (bits != 5 || stride) return hipErrorInvalidValue;
+    if (first) PBD_LAMBDA(5, true, false, true);
+    else PBD_LAMBDA(5, false, false, true);
+  } else if (bits == 5) {
     if (first) PBD_LAMBDA_S(5, true);
     else PBD_LAMBDA_S(5, false);
   } else {
@@ -967,29 +1035,32 @@ hipError_t launch_tile_lambda(hipStream_t st, int bits, uint32_t grid, bool stri
   return hipGetLastError();
 }
 
-template <int BITS, bool COLLIDE, int NS>
+template <int BITS, bool COLLIDE, int NS, bool PH>
 static void launch_delta_t(hipStream_t st, uint32_t grid, bool stride, const DevParams& P, const TileList& T,
                            const StepBuffers& B, int use_list, uint32_t flags, const float4* Q, const float4* QX,
                            float4* Qo, float4* QXo, const uint32_t* keys) {
-  if (stride)
-    hipLaunchKernelGGL((k_tile_delta<BITS, COLLIDE, NS, true>), dim3(grid), dim3(kTileThreads), 0, st, P, T, nlist(B),
-                       use_list, flags, Q, QX, B.LAM, keys, B.cstart, B.cend, Qo, QXo, B.DP, B.VS);
-  else
-    hipLaunchKernelGGL((k_tile_delta<BITS, COLLIDE, NS, false>), dim3(grid), dim3(kTileThreads), 0, st, P, T, nlist(B),
-                       use_list, flags, Q, QX, B.LAM, keys, B.cstart, B.cend, Qo, QXo, B.DP, B.VS);
+  if constexpr (!PH) {
+    if (stride) {
+      hipLaunchKernelGGL((k_tile_delta<BITS, COLLIDE, NS, true, false>), dim3(grid), dim3(kTileThreads), 0, st, P, T,
+                         nlist(B), use_list, flags, Q, QX, B.LAM, keys, B.cstart, B.cend, Qo, QXo, B.DP, B.VS);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_tile_delta<BITS, COLLIDE, NS, false, PH>), dim3(grid), dim3(kTileThreads), 0, st, P, T,
+                     nlist(B), use_list, flags, Q, QX, B.LAM, keys, B.cstart, B.cend, Qo, QXo, B.DP, B.VS);
 }
 
-template <int BITS>
+template <int BITS, bool PH>
 static void launch_delta_b(hipStream_t st, uint32_t grid, bool stride, const DevParams& P, const TileList& T,
                            const StepBuffers& B, int use_list, uint32_t flags, bool collide, const float4* Q,
                            const float4* QX, float4* Qo, float4* QXo, const uint32_t* keys) {
   const bool n4 = P.n_scorr == 4;  // the s_corr exponent of Macklin & Mueller 2013
   if (collide) {
-    if (n4) launch_delta_t<BITS, true, 4>(st, grid, stride, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
-    else launch_delta_t<BITS, true, 0>(st, grid, stride, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
+    if (n4) launch_delta_t<BITS, true, 4, PH>(st, grid, stride, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
+    else launch_delta_t<BITS, true, 0, PH>(st, grid, stride, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
   } else {
-    if (n4) launch_delta_t<BITS, false, 4>(st, grid, stride, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
-    else launch_delta_t<BITS, false, 0>(st, grid, stride, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
+    if (n4) launch_delta_t<BITS, false, 4, PH>(st, grid, stride, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
+    else launch_delta_t<BITS, false, 0, PH>(st, grid, stride, P, T, B, use_list, flags, Q, QX, Qo, QXo, keys);
   }
 }
 
@@ -997,8 +1068,14 @@ hipError_t launch_tile_delta(hipStream_t st, int bits, uint32_t grid, bool strid
                              const TileList& T, const StepBuffers& B, uint32_t flags, bool collide, bool use_list,
                              const float4* Q, const float4* QX, float4* Qo, float4* QXo, const uint32_t* keys) {
   const int ul = use_list ? 1 : 0;
-  if (bits == 5) launch_delta_b<5>(st, grid, stride, P, T, B, ul, flags, collide, Q, QX, Qo, QXo, keys);
-  else launch_delta_b<6>(st, grid, stride, P, T, B, ul, flags, collide, Q, QX, Qo, QXo, keys);
+  if (P.phased) {
+    if (bits != 5 || stride) return hipErrorInvalidValue;
+    launch_delta_b<5, true>(st, grid, false, P, T, B, ul, flags, collide, Q, QX, Qo, QXo, keys);
+  } else if (bits == 5) {
+    launch_delta_b<5, false>(st, grid, stride, P, T, B, ul, flags, collide, Q, QX, Qo, QXo, keys);
+  } else {
+    launch_delta_b<6, false>(st, grid, stride, P, T, B, ul, flags, collide, Q, QX, Qo, QXo, keys);
+  }
   return hipGetLastError();
 }
 
@@ -1007,12 +1084,17 @@ hipError_t launch_tile_omega(hipStream_t st, int bits, uint32_t grid, bool strid
                              bool xsph) {
   dim3 g(grid), b(kTileThreads);
   const NeighbourList L = nlist(B);
-#define PBD_OMEGA(BITS_, X_, S_) \
-  hipLaunchKernelGGL((k_tile_omega<BITS_, X_, S_>), g, b, 0, st, P, T, L, Q, B.VS, keys, B.cstart, B.cend, B.OM, B.V)
+#define PBD_OMEGA(BITS_, X_, S_, PH_)                                                                             \
+  hipLaunchKernelGGL((k_tile_omega<BITS_, X_, S_, PH_>), g, b, 0, st, P, T, L, Q, B.VS, keys, B.cstart, B.cend, B.OM, \
+                     B.V)
 #define PBD_OMEGA_S(BITS_, X_) \
-  if (stride) PBD_OMEGA(BITS_, X_, true); \
-  else PBD_OMEGA(BITS_, X_, false)
-  if (bits == 5) {
+  if (stride) PBD_OMEGA(BITS_, X_, true, false); \
+  else PBD_OMEGA(BITS_, X_, false, false)
+  if (P.phased) {
+    if (bits != 5 || stride) return hipErrorInvalidValue;
+    if (xsph) PBD_OMEGA(5, true, false, true);
+    else PBD_OMEGA(5, false, false, true);
+  } else if (bits == 5) {
     if (xsph) PBD_OMEGA_S(5, true);
     else PBD_OMEGA_S(5, false);
   } else {
@@ -1031,13 +1113,19 @@ hipError_t launch_tile_final(hipStream_t st, int bits, uint32_t grid, bool strid
   const NeighbourList L = nlist(B);
   const int ul = use_list ? 1 : 0;
   const bool vort = (flags & FF_VORT) != 0, xsph = (flags & FF_XSPH) != 0;
-#define PBD_FINAL(BITS_, V_, X_, S_)                                                                             \
-  hipLaunchKernelGGL((k_tile_final<BITS_, V_, X_, S_>), g, b, 0, st, P, T, L, ul, flags, Q, QX, B.VS, B.OM, keys, \
+#define PBD_FINAL(BITS_, V_, X_, S_, PH_)                                                                             \
+  hipLaunchKernelGGL((k_tile_final<BITS_, V_, X_, S_, PH_>), g, b, 0, st, P, T, L, ul, flags, Q, QX, B.VS, B.OM, keys, \
                      B.cstart, B.cend, B.CS, B.NCNT, B.X, B.P, B.V, B.C, B.F)
 #define PBD_FINAL_S(BITS_, V_, X_) \
-  if (stride) PBD_FINAL(BITS_, V_, X_, true); \
-  else PBD_FINAL(BITS_, V_, X_, false)
-  if (bits == 5) {
+  if (stride) PBD_FINAL(BITS_, V_, X_, true, false); \
+  else PBD_FINAL(BITS_, V_, X_, false, false)
+  if (P.phased) {
+    if (bits != 5 || stride) return hipErrorInvalidValue;
+    if (vort && xsph) PBD_FINAL(5, true, true, false, true);
+    else if (vort) PBD_FINAL(5, true, false, false, true);
+    else if (xsph) PBD_FINAL(5, false, true, false, true);
+    else PBD_FINAL(5, false, false, false, true);
+  } else if (bits == 5) {
     if (vort && xsph) PBD_FINAL_S(5, true, true);
     else if (vort) PBD_FINAL_S(5, true, false);
     else if (xsph) PBD_FINAL_S(5, false, true);

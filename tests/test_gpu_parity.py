@@ -309,23 +309,23 @@ def test_gpu_batch_matches_single_scenes(F, O):
 
 
 def test_gpu_batch_strided_tile_grid(F, O):
-    """256 scenes of 10x10x11 particles (281,600 in all) in 3.2-wide enclosures:
+    """256 scenes of 16x16x17 particles (1,114,112 in all) in 3.2-wide enclosures:
     each scene takes a 64-cell Morton slot, so the cell table bounds 2^21 tiles and
-    the tile grid min(tiles, particles) = 281,600 exceeds the 2^18 workgroups a
+    the tile grid min(tiles, particles) = 1,114,112 exceeds the 2^20 workgroups a
     direct grid may hold.  The neighbour passes then run on the capped grid, each
     workgroup taking every gridDim-th work item.  Three scenes step as the oracle
     steps them alone; every scene keeps sorted keys and finite state."""
     scenes = []
     for k in range(256):
-        p = small_dam(F, 10)
-        p.box_depth = 11
+        p = small_dam(F, 16)
+        p.box_depth = 17
         p.bounds_max[0] = p.bounds_max[1] = p.bounds_max[2] = 3.2
         p.relaxation = 300.0 + 2.0 * k
         p.box_seed_lo = 100 + k
         scenes.append(p)
     b = F.capi.Batch(scenes)
     total = sum(b.scene(s)[0] for s in range(256))
-    assert total == 281600
+    assert total == 1114112
     flush(f"[batch strided] {len(scenes)} scenes, {total} particles")
     b.compute()
     first = 0

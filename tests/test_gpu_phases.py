@@ -54,10 +54,13 @@ def test_gpu_friction_pair_matches_oracle(F, O, slip, phases):
     ctx.close()
 
 
-def test_gpu_mixed_step_matches_oracle(F, O):
+@pytest.mark.parametrize("tile_bits", [5, 0])
+def test_gpu_mixed_step_matches_oracle(F, O, tile_bits):
     """A 12^3 dam break with 30 % of its particles solid (seeded spheres of radius
-    0.1, static mu 0.5, kinetic 0.4): one step against the oracle."""
+    0.1, static mu 0.5, kinetic 0.4): one step against the oracle, on the LDS
+    tiles (phases in the neighbour lists) and on the per-particle kernels."""
     p = small_dam(F, 12)
+    p.tile_bits = tile_bits
     p.friction_static = 0.5
     p.friction_kinetic = 0.4
     p.box_solid_fraction = 0.3
@@ -67,7 +70,7 @@ def test_gpu_mixed_step_matches_oracle(F, O):
     assert 0.29 < ph.mean() < 0.4
     rng = np.random.default_rng(4)
     vel[:, :3] = rng.normal(0.0, 0.2, (pos.shape[0], 3)).astype(np.float32)
-    flush(f"mixed step: n={pos.shape[0]}, solid share {ph.mean():.3f}")
+    flush(f"mixed step (tile bits {tile_bits}): n={pos.shape[0]}, solid share {ph.mean():.3f}")
     ctx = _ctx(F, p, pos, vel, col, ph)
     ctx.compute()
     keys, ids = ctx.download("cell_ids"), ctx.download("particle_ids")
