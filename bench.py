@@ -138,6 +138,9 @@ def batched_vs_plain(F, n_scenes, steps, warmup):
     }
 
 
+GRADIENT_FRAMES = 5  # horizon of the reported gradient values
+
+
 def adjoint_leg(F, frames, k_ckpt, seed=5, stabilization=None):
     """adj/fwd on the 65,536-particle adjoint scene: the checkpointed forward
     (pbd_adj_forward), the segment recomputation and the reverse passes
@@ -201,13 +204,23 @@ def adjoint_leg(F, frames, k_ckpt, seed=5, stabilization=None):
     c.close()
     target = targets(frames)
     g, t_af, t_ab, loss, gxn = gradient(frames, k_ckpt, target, True)
+
+    def finite_or_none(x):
+        x = float(x)
+        return x if np.isfinite(x) else None
+
     growth = {}
+    short = None
     for fr in (1, 5, 10, 30):
         if fr < frames:
-            gs, _, _, _, gsn = gradient(fr, min(k_ckpt, fr), targets(fr), False)
-            growth[str(fr)] = float(np.linalg.norm(gs["v0"]) / gsn) if gsn > 0 else None
+            gs, _, _, ls, gsn = gradient(fr, min(k_ckpt, fr), targets(fr), False)
+            growth[str(fr)] = finite_or_none(np.linalg.norm(gs["v0"]) / gsn) if gsn > 0 else None
+            if fr == GRADIENT_FRAMES:
+                short = {"frames": fr, "loss": ls, "dL_dv0_norm": finite_or_none(np.linalg.norm(gs["v0"])),
+                         "dL_dv0_finite": bool(np.isfinite(gs["v0"]).all()),
+                         "dL_drho0": finite_or_none(gs["rho0"]), "dL_deps": finite_or_none(gs["eps"])}
     gn = float(np.linalg.norm(g["v0"]))
-    growth[str(frames)] = gn / gxn if gxn > 0 else None
+    growth[str(frames)] = finite_or_none(gn / gxn) if gxn > 0 else None
     return {
         "particles": n,
         "frames": frames,
@@ -218,14 +231,15 @@ def adjoint_leg(F, frames, k_ckpt, seed=5, stabilization=None):
         "adj_backward_s": t_ab,
         "adj_over_fwd": (t_af + t_ab) / t_plain,
         "loss": loss,
-        "dL_dv0_norm": gn,
-        "dL_dv0_finite": bool(np.isfinite(g["v0"]).all()),
-        "dL_drho0": g["rho0"],
-        "dL_deps": g["eps"],
+        "timed_gradient_finite": bool(np.isfinite(g["v0"]).all()),
+        "gradient": short,
         "gradient_growth": growth,
         "workload": "adjoint_64k: 64x32x32 lattice (65,536 particles) in a 6.4x1.6x1.6 box, the 32k dam-break "
-                    "parameters; loss = mean |x_T - target|^2, targets from rest density +5% and v0 ~ N(0, 0.1^2); "
-                    "gradient_growth = |dL/dv0| / |dL/dx_T| per horizon in frames",
+                    "parameters; loss = mean |x_T - target|^2, targets from rest density +5% and v0 ~ N(0, 0.1^2)",
+        "note": "adj/fwd is timed over `frames`; `gradient` is the same loss over a "
+                f"{GRADIENT_FRAMES}-frame horizon; gradient_growth = |dL/dv0| / |dL/dx_T| per horizon in "
+                "frames, null where the exact FP32 adjoint of this chaotic, colliding flow overflows "
+                "(around 30-50 frames)",
     }
 
 
@@ -403,11 +417,11 @@ def main():
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_leg(F, args.adjoint_frames, args.adjoint_checkpoint)
-        # the same scene without the particle-collision pass, whose tangent map stays within FP32 range
+        # the same scene without the particle-collision pass, whose tangent map grows slower
         nc = adjoint_leg(F, args.adjoint_frames, args.adjoint_checkpoint, stabilization=0)
         adjoint["without_particle_collisions"] = {k: nc[k] for k in (
-            "plain_forward_s", "adj_forward_s", "adj_backward_s", "adj_over_fwd", "loss", "dL_dv0_norm",
-            "dL_dv0_finite", "dL_drho0", "dL_deps", "gradient_growth")}
+            "plain_forward_s", "adj_forward_s", "adj_backward_s", "adj_over_fwd", "loss", "timed_gradient_finite",
+            "gradient", "gradient_growth")}
 
     line = {
         "metric": METRIC,
