@@ -145,6 +145,8 @@ typedef struct tape64 {
   double* lam;      /* iters x n */
   double* U;        /* 3n velocities after the update */
   double* OM;       /* 3n vorticity */
+  uint8_t* sol;     /* n: 1 for a solid particle (NULL: all fluid) */
+  double* Xq;       /* iters x 3n: x at each iteration's start (friction) */
 } tape64;
 
 static void tape_free(tape64* t) {
@@ -159,6 +161,8 @@ static void tape_free(tape64* t) {
   free(t->lam);
   free(t->U);
   free(t->OM);
+  free(t->sol);
+  free(t->Xq);
   memset(t, 0, sizeof(*t));
 }
 
@@ -166,6 +170,7 @@ typedef struct ctx64 {
   orc_derived d;
   k64 k;
   double rho0, eps, lo[3], hi[3];
+  double mu_s, mu_k; /* friction on contacts with a solid particle (0: none) */
   uint32_t n, iters, stab;
 } ctx64;
 
@@ -185,6 +190,8 @@ static int ctx_init(const pbd_params* p, double rho0, double eps, uint32_t n, ct
     c->lo[a] = (double)(c->d.bmin_f[a] + 0.001f);
     c->hi[a] = (double)(c->d.bmax_f[a] - 0.001f);
   }
+  c->mu_s = c->d.mu_s;
+  c->mu_k = c->d.mu_k;
   c->n = n;
   c->iters = c->d.solver_iters;
   c->stab = c->d.stab_iters;
@@ -262,9 +269,63 @@ static int build_lists(const ctx64* c, const uint32_t* keys, const uint32_t* st,
   return PBD_OK;
 }
 
+/* Friction of one contact with a solid particle (Macklin et al. 2014, eq.
+ * 24-25), as pass_collide in pbf_oracle.c: r = p_i - p_j (length len),
+ * u = (p_i - x_i) - (p_j - x_j); adds -0.5 f t to out, t the tangential part
+ * of u, f = 1 when |t| < mu_s d (d = diameter - len) else min(mu_k d / |t|, 1). */
+static void fric64(const ctx64* c, const double* r, double len, const double* u, double* out) {
+  const double ov = c->k.diam - len;
+  double nr[3], t[3];
+  for (int m = 0; m < 3; ++m) nr[m] = r[m] / len;
+  const double un = dot3(u, nr);
+  for (int m = 0; m < 3; ++m) t[m] = u[m] - un * nr[m];
+  const double tl = sqrt(dot3(t, t));
+  double f = 1.0;
+  if (!(tl < c->mu_s * ov)) f = fmin(c->mu_k * ov / tl, 1.0);
+  for (int m = 0; m < 3; ++m) out[m] -= 0.5 * f * t[m];
+}
+
+/* Adjoint of fric64's term F(r, u) in a pair term e . F: gr = J_r^T e and
+ * gu = J_u^T e, the derivative of the branch the forward took.  With n = r/len,
+ * un = u.n, t = u - un n, T = |t|, e_p = e - (e.n) n:
+ *   f = 1:  F = -t/2,  gu = -e_p/2,  gr = ((n.e) t + un e_p) / (2 len);
+ *   f = mu_k d/T:  F = -mu_k d t/(2T); with w = e - (e.t/T) t/T, w_p = w - (w.n) n,
+ *           gu = -f w_p/2,  gr = mu_k (e.t/T) n/2 + f ((n.w) t + un w_p) / (2 len). */
+static void fric64_adj(const ctx64* c, const double* r, double len, const double* u, const double* e, double* gr,
+                       double* gu) {
+  const double ov = c->k.diam - len;
+  double nr[3], t[3], ep[3];
+  for (int m = 0; m < 3; ++m) nr[m] = r[m] / len;
+  const double un = dot3(u, nr), ne = dot3(nr, e);
+  for (int m = 0; m < 3; ++m) {
+    t[m] = u[m] - un * nr[m];
+    ep[m] = e[m] - ne * nr[m];
+  }
+  const double tl = sqrt(dot3(t, t));
+  if (tl < c->mu_s * ov || c->mu_k * ov >= tl) {
+    for (int m = 0; m < 3; ++m) {
+      gu[m] = -0.5 * ep[m];
+      gr[m] = 0.5 / len * (ne * t[m] + un * ep[m]);
+    }
+    return;
+  }
+  const double f = c->mu_k * ov / tl;
+  double th[3], w[3];
+  for (int m = 0; m < 3; ++m) th[m] = t[m] / tl;
+  const double te = dot3(th, e);
+  for (int m = 0; m < 3; ++m) w[m] = e[m] - te * th[m];
+  const double nw = dot3(nr, w);
+  for (int m = 0; m < 3; ++m) {
+    const double wp = w[m] - nw * nr[m];
+    gu[m] = -0.5 * f * wp;
+    gr[m] = 0.5 * c->mu_k * te * nr[m] + 0.5 * f / len * (nw * t[m] + un * wp);
+  }
+}
+
 /* One FP64 step.  X, V, F: 3n state in/out (order before the sort on entry,
- * sorted on exit); orig carried through the sort.  t: record (NULL = none). */
-static int step64(const ctx64* c, double* X, double* V, double* F, uint32_t* orig, tape64* t) {
+ * sorted on exit); orig and phase (NULL: all fluid) carried through the sort.
+ * t: record (NULL = none). */
+static int step64(const ctx64* c, double* X, double* V, double* F, uint32_t* orig, uint32_t* phase, tape64* t) {
   const uint32_t n = c->n, iters = c->iters;
   const k64* k = &c->k;
   const double dt = c->d.dt, rho0 = c->rho0, eps = c->eps;
@@ -286,7 +347,10 @@ static int step64(const ctx64* c, double* X, double* V, double* F, uint32_t* ori
   uint32_t* st = (uint32_t*)malloc((size_t)c->d.table_size * 4);
   uint32_t* en = (uint32_t*)malloc((size_t)c->d.table_size * 4);
   uint32_t* ob = (uint32_t*)malloc((size_t)n * 4);
-  if (!Xp || !Pp || !Xs || !Ps || !lam || !dp || !pu || !om || !keys0 || !keys || !perm || !st || !en || !ob) {
+  double* fr3 = (double*)calloc((size_t)n * 3, 8);
+  uint8_t* sol = phase ? (uint8_t*)malloc((size_t)n) : NULL;
+  if (!Xp || !Pp || !Xs || !Ps || !lam || !dp || !pu || !om || !keys0 || !keys || !perm || !st || !en || !ob ||
+      !fr3 || (phase && !sol)) {
     rc = PBD_ERR_NOMEM;
     goto out;
   }
@@ -299,7 +363,8 @@ static int step64(const ctx64* c, double* X, double* V, double* F, uint32_t* ori
     t->lam = (double*)malloc((size_t)(iters ? iters : 1) * n * 8);
     t->U = (double*)malloc((size_t)n * 24);
     t->OM = (double*)malloc((size_t)n * 24);
-    if (!t->xin || !t->vin || !t->fin || !t->Q || !t->lam || !t->U || !t->OM) {
+    t->Xq = (double*)malloc((size_t)(iters ? iters : 1) * n * 24);
+    if (!t->xin || !t->vin || !t->fin || !t->Q || !t->lam || !t->U || !t->OM || !t->Xq) {
       rc = PBD_ERR_NOMEM;
       goto out;
     }
@@ -322,6 +387,11 @@ static int step64(const ctx64* c, double* X, double* V, double* F, uint32_t* ori
     ob[s] = orig[j];
   }
   memcpy(orig, ob, (size_t)n * 4);
+  if (phase) {
+    for (uint32_t s = 0; s < n; ++s) ob[s] = phase[perm[s]];
+    memcpy(phase, ob, (size_t)n * 4);
+    for (uint32_t s = 0; s < n; ++s) sol[s] = phase[s] == PBD_PHASE_SOLID;
+  }
   orc_cell_ranges(n, keys, c->d.table_size, st, en);
   rc = build_lists(c, keys, st, en, T);
   if (rc != PBD_OK) goto out;
@@ -330,13 +400,17 @@ static int step64(const ctx64* c, double* X, double* V, double* F, uint32_t* ori
   const uint32_t* ncon = T->ncon;
 
   for (uint32_t it = 0; it < iters; ++it) {
-    if (t) memcpy(&t->Q[(size_t)it * 3 * n], Ps, (size_t)n * 24);
+    if (t) {
+      memcpy(&t->Q[(size_t)it * 3 * n], Ps, (size_t)n * 24);
+      memcpy(&t->Xq[(size_t)it * 3 * n], Xs, (size_t)n * 24);
+    }
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
       const uint32_t i = (uint32_t)ii;
       double A = k->w0, a[3] = {0, 0, 0}, B = 0.0;
       for (uint32_t e = nbo[i]; e < nbo[i + 1]; ++e) {
         const uint32_t j = nbl[e];
+        if (sol && sol[j]) continue; /* fluid neighbours only */
         double r[3];
         pair64 q;
         sub3(&Ps[3 * i], &Ps[3 * j], r);
@@ -345,34 +419,44 @@ static int step64(const ctx64* c, double* X, double* V, double* F, uint32_t* ori
         for (int m = 0; m < 3; ++m) a[m] += q.G[m];
         B += dot3(q.G, q.G);
       }
-      lam[i] = -(A / rho0 - 1.0) / ((dot3(a, a) + B) / (rho0 * rho0) + eps);
+      lam[i] = (sol && sol[i]) ? 0.0 : -(A / rho0 - 1.0) / ((dot3(a, a) + B) / (rho0 * rho0) + eps);
     }
     if (t) memcpy(&t->lam[(size_t)it * n], lam, (size_t)n * 8);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
       const uint32_t i = (uint32_t)ii;
-      double acc[3] = {0, 0, 0}, push[3] = {0, 0, 0};
+      const int si = sol && sol[i];
+      double acc[3] = {0, 0, 0}, push[3] = {0, 0, 0}, fr[3] = {0, 0, 0};
       for (uint32_t e = nbo[i]; e < nbo[i + 1]; ++e) {
         const uint32_t j = nbl[e];
+        const int sj = sol && sol[j];
         double r[3];
         pair64 q;
         sub3(&Ps[3 * i], &Ps[3 * j], r);
         pair_eval(k, r, &q);
-        const double s = lam[i] + lam[j] + scorr64(k, q.W);
-        for (int m = 0; m < 3; ++m) acc[m] += s * q.G[m];
+        if (!si && !sj) { /* delta p between fluid particles only */
+          const double s = lam[i] + lam[j] + scorr64(k, q.W);
+          for (int m = 0; m < 3; ++m) acc[m] += s * q.G[m];
+        }
         if (c->stab && e - nbo[i] < ncon[i] && q.r < k->diam && q.r > 0.00001) {
           const double f = k->kc * (k->diam - q.r) / q.r;
           for (int m = 0; m < 3; ++m) push[m] += f * r[m];
+          if (c->mu_s > 0.0 && (si || sj)) {
+            double u[3];
+            for (int m = 0; m < 3; ++m) u[m] = (Ps[3 * i + m] - Xs[3 * i + m]) - (Ps[3 * j + m] - Xs[3 * j + m]);
+            fric64(c, r, q.r, u, fr);
+          }
         }
       }
       for (int m = 0; m < 3; ++m) {
         dp[3 * i + m] = acc[m] / rho0;
         pu[3 * i + m] = push[m];
+        fr3[3 * i + m] = fr[m];
       }
     }
     for (uint32_t i = 0; i < n; ++i)
       for (int m = 0; m < 3; ++m) {
-        double p = Ps[3 * i + m] + pu[3 * i + m] + dp[3 * i + m];
+        double p = Ps[3 * i + m] + pu[3 * i + m] + fr3[3 * i + m] + dp[3 * i + m];
         if (p < c->lo[m]) p = c->lo[m];
         else if (p > c->hi[m]) p = c->hi[m];
         Ps[3 * i + m] = p;
@@ -394,6 +478,7 @@ static int step64(const ctx64* c, double* X, double* V, double* F, uint32_t* ori
     double w[3] = {0, 0, 0};
     for (uint32_t e = nbo[i]; e < nbo[i + 1]; ++e) {
       const uint32_t j = nbl[e];
+      if (sol && (sol[i] || sol[j])) continue; /* fluid pairs only */
       double r[3], u[3], cr[3];
       pair64 q;
       sub3(&X[3 * i], &X[3 * j], r);
@@ -413,6 +498,7 @@ static int step64(const ctx64* c, double* X, double* V, double* F, uint32_t* ori
     double eta[3] = {0, 0, 0}, xs[3] = {0, 0, 0};
     for (uint32_t e = nbo[i]; e < nbo[i + 1]; ++e) {
       const uint32_t j = nbl[e];
+      if (sol && (sol[i] || sol[j])) continue; /* fluid pairs only */
       double r[3];
       pair64 q;
       sub3(&X[3 * i], &X[3 * j], r);
@@ -437,6 +523,8 @@ static int step64(const ctx64* c, double* X, double* V, double* F, uint32_t* ori
   if (t) {
     t->perm = perm;
     perm = NULL;
+    t->sol = sol;
+    sol = NULL;
   }
 out:
   if (!t) tape_free(&own);
@@ -454,6 +542,8 @@ out:
   free(st);
   free(en);
   free(ob);
+  free(fr3);
+  free(sol);
   return rc;
 }
 
@@ -467,6 +557,7 @@ static int reverse64(const ctx64* c, const tape64* t, double* gX, double* gV, do
   const uint32_t *nbo = t->nbo, *nbl = t->nbl, *ncon = t->ncon;
   const double* X = &t->Q[(size_t)iters * 3 * n];
   const double *U = t->U, *OM = t->OM;
+  const uint8_t* sol = t->sol; /* phases: fluid sums over fluid pairs only; friction on solid contacts */
   double* Ub = (double*)calloc((size_t)n * 3, 8);
   double* Pb = (double*)calloc((size_t)n * 3, 8);
   double* Pn = (double*)calloc((size_t)n * 3, 8);
@@ -480,8 +571,9 @@ static int reverse64(const ctx64* c, const tape64* t, double* gX, double* gV, do
   double* Rb = (double*)calloc((size_t)n, 8);
   double* gr = (double*)calloc((size_t)n, 8); /* per-particle d L / d rho0 terms */
   double* ge = (double*)calloc((size_t)n, 8); /* per-particle d L / d eps terms */
+  double* XF = (double*)calloc((size_t)n * 3, 8); /* friction's x adjoint of one iteration */
   int rc = PBD_OK;
-  if (!Ub || !Pb || !Pn || !Xb || !Eb || !Ob || !DPb || !PUb || !Ab || !Bb || !Rb || !gr || !ge) {
+  if (!Ub || !Pb || !Pn || !Xb || !Eb || !Ob || !DPb || !PUb || !Ab || !Bb || !Rb || !gr || !ge || !XF) {
     rc = PBD_ERR_NOMEM;
     goto out;
   }
@@ -494,6 +586,7 @@ static int reverse64(const ctx64* c, const tape64* t, double* gX, double* gV, do
     double ub[3] = {gvi[0], gvi[1], gvi[2]}, xb[3] = {0, 0, 0}, eta[3] = {0, 0, 0}, xs[3] = {0, 0, 0};
     for (uint32_t e = nbo[i]; e < nbo[i + 1]; ++e) {
       const uint32_t j = nbl[e];
+      if (sol && (sol[i] || sol[j])) continue;
       double r[3], du[3], dg[3];
       pair64 q;
       sub3(&X[3 * i], &X[3 * j], r);
@@ -539,6 +632,7 @@ static int reverse64(const ctx64* c, const tape64* t, double* gX, double* gV, do
     double mb = 0.0, xb[3] = {0, 0, 0};
     for (uint32_t e = nbo[i]; e < nbo[i + 1]; ++e) {
       const uint32_t j = nbl[e];
+      if (sol && (sol[i] || sol[j])) continue;
       double r[3], es[3];
       pair64 q;
       sub3(&X[3 * i], &X[3 * j], r);
@@ -562,6 +656,7 @@ static int reverse64(const ctx64* c, const tape64* t, double* gX, double* gV, do
     double ub[3] = {0, 0, 0}, xb[3] = {0, 0, 0};
     for (uint32_t e = nbo[i]; e < nbo[i + 1]; ++e) {
       const uint32_t j = nbl[e];
+      if (sol && (sol[i] || sol[j])) continue;
       double r[3], os[3], cr[3], du[3];
       pair64 q;
       sub3(&X[3 * i], &X[3 * j], r);
@@ -585,6 +680,7 @@ static int reverse64(const ctx64* c, const tape64* t, double* gX, double* gV, do
     const double* Q = &t->Q[(size_t)it1 * 3 * n];
     const double* Qn = &t->Q[(size_t)(it1 + 1) * 3 * n];
     const double* lam = &t->lam[(size_t)it1 * n];
+    const double* Xq = &t->Xq[(size_t)it1 * 3 * n];
     for (uint32_t i = 0; i < n; ++i)
       for (int m = 0; m < 3; ++m) {
         const double q = Qn[3 * i + m];
@@ -596,14 +692,37 @@ static int reverse64(const ctx64* c, const tape64* t, double* gX, double* gV, do
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
       const uint32_t i = (uint32_t)ii;
-      double A = k->w0, a[3] = {0, 0, 0}, B = 0.0, lb = 0.0, pn[3], dpi[3] = {0, 0, 0};
+      double A = k->w0, a[3] = {0, 0, 0}, B = 0.0, lb = 0.0, pn[3], dpi[3] = {0, 0, 0}, xf[3] = {0, 0, 0};
       for (int m = 0; m < 3; ++m) pn[m] = DPb[3 * i + m];
+      const int si = sol && sol[i];
       for (uint32_t e = nbo[i]; e < nbo[i + 1]; ++e) {
         const uint32_t j = nbl[e];
+        const int sj = sol && sol[j];
         double r[3], dd[3];
         pair64 q;
         sub3(&Q[3 * i], &Q[3 * j], r);
         pair_eval(k, r, &q);
+        const int contact = c->stab && e - nbo[i] < ncon[i] && q.r < k->diam && q.r > 0.00001;
+        if (contact && c->mu_s > 0.0 && (si || sj)) {
+          double u[3], ef[3], gr3[3], gu3[3];
+          for (int m = 0; m < 3; ++m) {
+            u[m] = (Q[3 * i + m] - Xq[3 * i + m]) - (Q[3 * j + m] - Xq[3 * j + m]);
+            ef[m] = DPb[3 * i + m] - DPb[3 * j + m];
+          }
+          fric64_adj(c, r, q.r, u, ef, gr3, gu3);
+          for (int m = 0; m < 3; ++m) {
+            pn[m] += gr3[m] + gu3[m];
+            xf[m] -= gu3[m];
+          }
+        }
+        if (contact) {
+          double pd[3];
+          sub3(&PUb[3 * i], &PUb[3 * j], pd);
+          const double phi = (k->diam - q.r) / q.r;
+          const double rr = dot3(r, pd) * k->diam / (q.r2 * q.r);
+          for (int m = 0; m < 3; ++m) pn[m] += k->kc * (phi * pd[m] - rr * r[m]);
+        }
+        if (si || sj) continue; /* density terms between fluid particles only */
         A += q.W;
         for (int m = 0; m < 3; ++m) a[m] += q.G[m];
         B += dot3(q.G, q.G);
@@ -615,13 +734,6 @@ static int reverse64(const ctx64* c, const tape64* t, double* gX, double* gV, do
         hmul_add(k, &q, r, dd, S / rho0, pn);
         const double ds = dscorr64(k, q.W) * gd / rho0;
         for (int m = 0; m < 3; ++m) pn[m] += ds * q.Wg[m];
-        if (c->stab && e - nbo[i] < ncon[i] && q.r < k->diam && q.r > 0.00001) {
-          double pd[3];
-          sub3(&PUb[3 * i], &PUb[3 * j], pd);
-          const double phi = (k->diam - q.r) / q.r;
-          const double rr = dot3(r, pd) * k->diam / (q.r2 * q.r);
-          for (int m = 0; m < 3; ++m) pn[m] += k->kc * (phi * pd[m] - rr * r[m]);
-        }
       }
       gr[i] += -dot3(&DPb[3 * i], dpi) / rho0;
       const double s2 = (dot3(a, a) + B) / (rho0 * rho0);
@@ -633,8 +745,14 @@ static int reverse64(const ctx64* c, const tape64* t, double* gX, double* gV, do
       Rb[i] = cb / rho0;
       for (int m = 0; m < 3; ++m) Ab[3 * i + m] = 2.0 * db * a[m] / (rho0 * rho0);
       Bb[i] = db / (rho0 * rho0);
-      for (int m = 0; m < 3; ++m) Pn[3 * i + m] = pn[m];
+      for (int m = 0; m < 3; ++m) {
+        Pn[3 * i + m] = pn[m];
+        XF[3 * i + m] = xf[m];
+      }
     }
+    /* friction reads x of this iteration: its adjoint joins x's before it */
+    if (sol && c->mu_s > 0.0)
+      for (size_t m = 0; m < (size_t)n * 3; ++m) Xb[m] += XF[m];
     /* R2 */
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
@@ -642,6 +760,7 @@ static int reverse64(const ctx64* c, const tape64* t, double* gX, double* gV, do
       double pn[3] = {0, 0, 0};
       for (uint32_t e = nbo[i]; e < nbo[i + 1]; ++e) {
         const uint32_t j = nbl[e];
+        if (sol && (sol[i] || sol[j])) continue;
         double r[3], da[3];
         pair64 q;
         sub3(&Q[3 * i], &Q[3 * j], r);
@@ -692,6 +811,7 @@ out:
   free(Rb);
   free(gr);
   free(ge);
+  free(XF);
   return rc;
 }
 
@@ -707,7 +827,7 @@ static void to4(uint32_t n, const double* a3, double* a4) {
 }
 
 int orc64_forward(const pbd_params* p, double rho0, double eps, uint32_t n, uint32_t nsub, double* x4, double* v4,
-                  double* f4, uint32_t* orig) {
+                  double* f4, uint32_t* orig, uint32_t* phase) {
   ctx64 c;
   int rc = ctx_init(p, rho0, eps, n, &c);
   if (rc != PBD_OK || n == 0) return rc;
@@ -721,7 +841,7 @@ int orc64_forward(const pbd_params* p, double rho0, double eps, uint32_t n, uint
   to3(n, x4, X);
   to3(n, v4, V);
   to3(n, f4, F);
-  for (uint32_t s = 0; s < nsub && rc == PBD_OK; ++s) rc = step64(&c, X, V, F, orig, NULL);
+  for (uint32_t s = 0; s < nsub && rc == PBD_OK; ++s) rc = step64(&c, X, V, F, orig, phase, NULL);
   if (rc == PBD_OK) {
     to4(n, X, x4);
     to4(n, V, v4);
@@ -735,8 +855,8 @@ out:
 }
 
 int orc64_gradient(const pbd_params* p, double rho0, double eps, uint32_t n, uint32_t nsub, const double* x4,
-                   const double* v4, const double* f4, const double* gx4, const double* gv4, double* gx0,
-                   double* gv0, double* grho0, double* geps) {
+                   const double* v4, const double* f4, const uint32_t* phase0, const double* gx4, const double* gv4,
+                   double* gx0, double* gv0, double* grho0, double* geps) {
   ctx64 c;
   int rc = ctx_init(p, rho0, eps, n, &c);
   if (rc != PBD_OK) return rc;
@@ -751,7 +871,8 @@ int orc64_gradient(const pbd_params* p, double rho0, double eps, uint32_t n, uin
   double* gV = (double*)malloc((size_t)n * 24);
   double* gF = (double*)calloc((size_t)n * 3, 8);
   uint32_t* orig = (uint32_t*)malloc((size_t)n * 4);
-  if (!tp || !X || !V || !F || !gX || !gV || !gF || !orig) {
+  uint32_t* phase = phase0 ? (uint32_t*)malloc((size_t)n * 4) : NULL;
+  if (!tp || !X || !V || !F || !gX || !gV || !gF || !orig || (phase0 && !phase)) {
     rc = PBD_ERR_NOMEM;
     goto out;
   }
@@ -759,7 +880,8 @@ int orc64_gradient(const pbd_params* p, double rho0, double eps, uint32_t n, uin
   to3(n, v4, V);
   to3(n, f4, F);
   for (uint32_t i = 0; i < n; ++i) orig[i] = i;
-  for (uint32_t s = 0; s < nsub && rc == PBD_OK; ++s) rc = step64(&c, X, V, F, orig, &tp[s]);
+  if (phase) memcpy(phase, phase0, (size_t)n * 4);
+  for (uint32_t s = 0; s < nsub && rc == PBD_OK; ++s) rc = step64(&c, X, V, F, orig, phase, &tp[s]);
   if (rc != PBD_OK) goto out;
   to3(n, gx4, gX);
   to3(n, gv4, gV);
@@ -779,5 +901,6 @@ out:
   free(gV);
   free(gF);
   free(orig);
+  free(phase);
   return rc;
 }

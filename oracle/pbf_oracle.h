@@ -122,14 +122,17 @@ int orc_step(const pbd_params* p, orc_io* io);
  * Textbook variant, collisionType 0, at most one stabilization pass, all
  * stages on (else PBD_ERR_UNSUPPORTED); a neighbour list reaching the cap is
  * PBD_ERR_UNSUPPORTED as well.  State: n x 4 doubles (w ignored).  rho0 <= 0
- * and eps < 0 select the values derived from the parameters. */
+ * and eps < 0 select the values derived from the parameters.  phase: n phases
+ * (PBD_PHASE_*; NULL: all fluid), carried through the sorts like orig; solid
+ * particles and friction as in orc_step. */
 int orc64_forward(const pbd_params* p, double rho0, double eps, uint32_t n, uint32_t nsub, double* x4,
-                  double* v4, double* f4, uint32_t* orig);
+                  double* v4, double* f4, uint32_t* orig, uint32_t* phase);
 /* dL/dx0, dL/dv0 (order of the initial state), dL/drho0, dL/deps of a loss with
- * adjoints gx4, gv4 of the final positions and velocities (final order). */
+ * adjoints gx4, gv4 of the final positions and velocities (final order);
+ * phase0: the initial phases (NULL: all fluid). */
 int orc64_gradient(const pbd_params* p, double rho0, double eps, uint32_t n, uint32_t nsub, const double* x4,
-                   const double* v4, const double* f4, const double* gx4, const double* gv4, double* gx0,
-                   double* gv0, double* grho0, double* geps);
+                   const double* v4, const double* f4, const uint32_t* phase0, const double* gx4,
+                   const double* gv4, double* gx0, double* gv0, double* grho0, double* geps);
 
 /* Number of OpenMP threads the oracle uses (1 when built without OpenMP). */
 int orc_threads(void);

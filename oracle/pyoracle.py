@@ -91,10 +91,10 @@ def lib() -> C.CDLL:
         L.orc_step.argtypes = [C.POINTER(PbdParams), C.POINTER(OrcIo)]
         L.orc_step.restype = C.c_int
         L.orc64_forward.argtypes = [C.POINTER(PbdParams), C.c_double, C.c_double, C.c_uint32, C.c_uint32,
-                                    _dp, _dp, _dp, _up]
+                                    _dp, _dp, _dp, _up, _up]
         L.orc64_forward.restype = C.c_int
         L.orc64_gradient.argtypes = [C.POINTER(PbdParams), C.c_double, C.c_double, C.c_uint32, C.c_uint32,
-                                     _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+                                     _dp, _dp, _dp, _up, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc64_gradient.restype = C.c_int
         L.orc_threads.argtypes = []
         L.orc_threads.restype = C.c_int
@@ -224,8 +224,9 @@ def _state64(a, n):
     return out
 
 
-def forward64(p: PbdParams, x, v=None, f=None, nsub: int = 1, rho0: float = 0.0, eps: float = -1.0):
-    """FP64 forward of nsub steps (textbook, collisionType 0) from state x, v, f.
+def forward64(p: PbdParams, x, v=None, f=None, nsub: int = 1, rho0: float = 0.0, eps: float = -1.0, phase=None):
+    """FP64 forward of nsub steps (textbook, collisionType 0) from state x, v, f
+    (and initial phases, None: all fluid).
 
     Returns (x, v, f, orig): the final state in its sorted order and, per final
     slot, the index of the particle in the initial state.  rho0 <= 0 / eps < 0
@@ -234,27 +235,30 @@ def forward64(p: PbdParams, x, v=None, f=None, nsub: int = 1, rho0: float = 0.0,
     n = x.shape[0]
     X, V, F = _state64(x, n), _state64(v, n), _state64(f, n)
     orig = np.arange(n, dtype=np.uint32)
+    ph = None if phase is None else np.ascontiguousarray(phase, dtype=np.uint32).copy()
     rc = lib().orc64_forward(C.byref(p), float(rho0), float(eps), n, int(nsub), _ptr(X, _dp), _ptr(V, _dp),
-                             _ptr(F, _dp), _ptr(orig, _up))
+                             _ptr(F, _dp), _ptr(orig, _up), None if ph is None else _ptr(ph, _up))
     if rc != 0:
         raise RuntimeError(f"orc64_forward failed ({rc})")
     return X[:, :3].copy(), V[:, :3].copy(), F[:, :3].copy(), orig
 
 
-def gradient64(p: PbdParams, x, v, gx, gv, f=None, nsub: int = 1, rho0: float = 0.0, eps: float = -1.0):
+def gradient64(p: PbdParams, x, v, gx, gv, f=None, nsub: int = 1, rho0: float = 0.0, eps: float = -1.0,
+               phase=None):
     """Reverse mode of forward64: given dL/dx, dL/dv of the final state (final
     order), returns dict(x0, v0: (n, 3) in the initial order, rho0, eps)."""
     x = np.asarray(x)
     n = x.shape[0]
     X, V, F = _state64(x, n), _state64(v, n), _state64(f, n)
     GX, GV = _state64(gx, n), _state64(gv, n)
+    ph = None if phase is None else np.ascontiguousarray(phase, dtype=np.uint32)
     gx0 = np.zeros((n, 4))
     gv0 = np.zeros((n, 4))
     gr = C.c_double(0.0)
     ge = C.c_double(0.0)
     rc = lib().orc64_gradient(C.byref(p), float(rho0), float(eps), n, int(nsub), _ptr(X, _dp), _ptr(V, _dp),
-                              _ptr(F, _dp), _ptr(GX, _dp), _ptr(GV, _dp), _ptr(gx0, _dp), _ptr(gv0, _dp),
-                              C.byref(gr), C.byref(ge))
+                              _ptr(F, _dp), None if ph is None else _ptr(ph, _up), _ptr(GX, _dp), _ptr(GV, _dp),
+                              _ptr(gx0, _dp), _ptr(gv0, _dp), C.byref(gr), C.byref(ge))
     if rc != 0:
         raise RuntimeError(f"orc64_gradient failed ({rc})")
     return dict(x0=gx0[:, :3].copy(), v0=gv0[:, :3].copy(), rho0=gr.value, eps=ge.value)
