@@ -243,6 +243,67 @@ def adjoint_leg(F, frames, k_ckpt, seed=5, stabilization=None):
     }
 
 
+def adjoint_mixed_leg(F, frames=4, k_ckpt=2, seed=11):
+    """Adjoint of the unified-phase scene at 1M particles: the dam_break_1m
+    lattice with 30 % granular solid spheres of radius 0.1 and friction (mu_s 0.5,
+    mu_k 0.4), gradient with respect to the initial velocities.  Loss: mean
+    squared distance of the final positions to targets 0.01 away (seeded);
+    `frames` frames with checkpoints every k_ckpt, so the earlier segments are
+    recomputed.  adj/fwd against the plain graph-launched forward of the same frames."""
+    p = F.load_params(F.config_path("dam_break_1m"))
+    p.friction_static = 0.5
+    p.friction_kinetic = 0.4
+    p.box_solid_fraction = 0.3
+    p.box_solid_radius = 0.1
+    pos, _, _ = F.generate_box(p)
+    ph = F.capi.generate_phases(p)
+    n = pos.shape[0]
+    q = p.copy()
+    q.box_spawn_in_the_beginning = 0
+
+    def context():
+        c = F.Context(q, n)
+        c.add_particles(pos)
+        c.upload("phases", ph)
+        c.step(0, graph=True)  # capture the step graph outside the timed regions
+        c.sync()
+        return c
+
+    c = context()
+    t0 = time.perf_counter()
+    c.step(frames, graph=True)
+    c.sync()
+    t_plain = time.perf_counter() - t0
+    c.close()
+    c = context()
+    t0 = time.perf_counter()
+    c.adj_forward(frames, k_ckpt)
+    c.sync()
+    t_af = time.perf_counter() - t0
+    xT = c.download("positions")[:, :3].astype(np.float64)
+    rng = np.random.default_rng(seed)
+    target = xT + rng.normal(0.0, 0.01, xT.shape)
+    gx = 2.0 * (xT - target) / n
+    t0 = time.perf_counter()
+    g = c.adj_backward(gx, None)
+    t_ab = time.perf_counter() - t0
+    c.close()
+    return {
+        "workload": f"dam_break_1m lattice ({n} particles), {100 * float(ph.mean()):.1f} % granular solid spheres of "
+                    "radius 0.1, friction mu_s 0.5 / mu_k 0.4; loss = mean |x_T - target|^2, targets 0.01 away",
+        "particles": n,
+        "frames": frames,
+        "substeps_per_frame": p.substeps,
+        "checkpoint_every_frames": k_ckpt,
+        "plain_forward_s": t_plain,
+        "adj_forward_s": t_af,
+        "adj_backward_s": t_ab,
+        "adj_over_fwd": (t_af + t_ab) / t_plain,
+        "dL_dv0_norm": float(np.linalg.norm(g["v0"])),
+        "dL_dv0_finite": bool(np.isfinite(g["v0"]).all()),
+    }
+
+
 def large_scene(F, steps, warmup, config="dam_break_16m"):
     """The step at 16.7M particles (256^3 lattice), whose ~6 GB working set
     exceeds the 256 MB Infinity Cache that holds most of the 1M scene; with
@@ -354,7 +415,9 @@ def main():
     if args.adjoint_only:
         import pbdfluid as F
 
-        print(json.dumps(adjoint_leg(F, args.adjoint_frames, args.adjoint_checkpoint)), flush=True)
+        out = adjoint_leg(F, args.adjoint_frames, args.adjoint_checkpoint)
+        out["unified_phase_1m"] = adjoint_mixed_leg(F)
+        print(json.dumps(out), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the step runs on a single "
@@ -422,6 +485,7 @@ def main():
         adjoint["without_particle_collisions"] = {k: nc[k] for k in (
             "plain_forward_s", "adj_forward_s", "adj_backward_s", "adj_over_fwd", "loss", "timed_gradient_finite",
             "gradient", "gradient_growth")}
+        adjoint["unified_phase_1m"] = adjoint_mixed_leg(F)
 
     line = {
         "metric": METRIC,

@@ -307,12 +307,16 @@ int pbd_write_frame_arrays(const char* path, int format, uint32_t frame, uint32_
  * with respect to the initial velocities, the rest density and the relaxation
  * eps (Density.relaxation).  Supported: kernelVariant textbook, collisionType 0
  * with at most one stabilization iteration, every Active.* stage on, plain
- * (not batched) contexts; otherwise PBD_ERR_UNSUPPORTED.
+ * (not batched) contexts; otherwise PBD_ERR_UNSUPPORTED.  Solid particles and
+ * friction (PBD_BUF_PHASES, Collision.friction*) are differentiated too: the
+ * phase masks, and the friction through p* and x of each iteration (the
+ * derivative of the sticking or sliding branch the forward took).
  *
  * pbd_adj_forward runs the frames (one graph launch per substep) and keeps a
- * checkpoint of the state (positions, velocities, external forces; 48 B per
- * particle) at the start of every k_ckpt-th frame.  The context is left in the
- * final state, in its final particle order. */
+ * checkpoint of the state (positions, velocities, external forces, and the
+ * phases of a phased context; 48 B per particle, +4) at the start of every
+ * k_ckpt-th frame.  The context is left in the final state, in its final
+ * particle order. */
 int pbd_adj_forward(pbd_ctx* ctx, uint32_t frames, uint32_t k_ckpt);
 /* For each particle of the final order (count <= number of particles), its
  * index in the order of the state pbd_adj_forward started from. */
@@ -321,7 +325,8 @@ int pbd_adj_ids(pbd_ctx* ctx, uint32_t* initial_index, size_t count);
  * Outputs (NULL = not wanted): dL_dv0 and dL_dx0, float4 per particle in the
  * initial order (w = 0); dL_drho0 and dL_deps, scalars.  Each segment of
  * k_ckpt frames is recomputed from its checkpoint while its steps are recorded
- * (about 104 + 20 x solverIterations bytes per particle and step), then the
+ * (about 104 + 20 x solverIterations bytes per particle and step; phases add 4,
+ * friction 16 x solverIterations), then the
  * reverse kernels run over the same sorted neighbour structure.  The context
  * is left as pbd_adj_forward left it.  PBD_ERR_UNSUPPORTED if a neighbour list
  * reached maxNeighboursPerParticle (a capped neighbour relation is not

@@ -84,6 +84,37 @@ __device__ __forceinline__ float3 hmul(const AdjPair& q, float3 r, float3 v) {
   return make_float3(q.gs * v.x - rv * r.x, q.gs * v.y - rv * r.y, q.gs * v.z - rv * r.z);
 }
 
+// Adjoint of the friction of one contact with a solid particle (friction_pair in
+// particle_ops.hpp; Macklin et al. 2014, eq. 24-25) in a pair term e . F(r, u),
+// r = p*_i - p*_j (length len, 1/len = il), u = (p*_i - x_i) - (p*_j - x_j):
+// gr = J_r^T e, gu = J_u^T e, the derivative of the branch the forward took.
+// With n = r/len, un = u.n, t = u - un n, T = |t|, d = diameter - len,
+// e_p = e - (e.n) n:
+//   sticking (F = -t/2):  gu = -e_p/2,  gr = ((n.e) t + un e_p) il/2;
+//   sliding (F = -f t/2, f = mu_k d/T < 1): w = e - (e.t/T) t/T, w_p = w - (w.n) n,
+//     gu = -f w_p/2,  gr = mu_k (e.t/T) n/2 + f ((n.w) t + un w_p) il/2.
+__device__ __forceinline__ void friction_adj(const DevParams& P, float3 r, float len, float il, float3 u, float3 e,
+                                             float3& gr, float3& gu) {
+  const float3 nr = il * r;
+  const float un = dot(u, nr), ne = dot(nr, e);
+  const float3 t = u - un * nr, ep = e - ne * nr;
+  const float tl = sqrtf(dot(t, t));
+  const float d = P.diameter - len;
+  if (tl < P.mu_s * d || P.mu_k * d >= tl) {
+    gu = -0.5f * ep;
+    gr = (0.5f * il) * (ne * t + un * ep);
+    return;
+  }
+  const float f = P.mu_k * d / tl;
+  const float3 th = (1.0f / tl) * t;
+  const float te = dot(th, e);
+  const float3 w = e - te * th;
+  const float nw = dot(nr, w);
+  const float3 wp = w - nw * nr;
+  gu = (-0.5f * f) * wp;
+  gr = (0.5f * P.mu_k * te) * nr + (0.5f * f * il) * (nw * t + un * wp);
+}
+
 // s_corr(W) = -k (W / W(dq))^n and its derivative in W
 __device__ __forceinline__ float scorr_of(const DevParams& P, float W) { return -P.k_scorr * ipow(W * P.inv_w_dq, P.n_scorr); }
 __device__ __forceinline__ float dscorr_of(const DevParams& P, float W) {
@@ -187,8 +218,11 @@ k_adj_a(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ 
   const float3 ui = f3(U[s]), gvi = f3(GV[s]), oi = f3(oi4);
   const CellFrame f = make_frame(P, keys[s]);
   const float cv = P.c_visc * P.vol;
+  const uint32_t* __restrict__ ph = P.phs;  // phases: fluid pairs only
+  const bool si = ph && ph[s];
   float3 ub = make_float3(0.f, 0.f, 0.f), xb = ub, eta = ub, xs = ub;
   const uint32_t cnt = adj_visit(P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
+        if (si || (ph && ph[j])) return;
         const float4 qj = Q[j];
         const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
         const AdjPair q = adj_pair(P, dot(r, r));
@@ -230,7 +264,10 @@ k_adj_b(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ 
   const CellFrame f = make_frame(P, keys[s]);
   float mb = 0.0f;
   float3 xb = make_float3(0.f, 0.f, 0.f);
+  const uint32_t* __restrict__ ph = P.phs;  // phases: fluid pairs only
+  const bool si = ph && ph[s];
   adj_visit(P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
+                     if (si || (ph && ph[j])) return;
                      const float4 qj = Q[j];
                      const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
                      const AdjPair q = adj_pair(P, dot(r, r));
@@ -263,7 +300,10 @@ k_adj_c(DevParams P, uint32_t stab, const float4* __restrict__ Q, const uint32_t
   const float3 ui = f3(U[s]), obi = f3(Ob[s]);
   const CellFrame f = make_frame(P, key);
   float3 ub = make_float3(0.f, 0.f, 0.f), xb = ub;
+  const uint32_t* __restrict__ ph = P.phs;  // phases: fluid pairs only
+  const bool si = ph && ph[s];
   adj_visit(P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
+                     if (si || (ph && ph[j])) return;
                      const float4 qj = Q[j];
                      const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
                      const AdjPair q = adj_pair(P, dot(r, r));
@@ -288,8 +328,9 @@ k_adj_c(DevParams P, uint32_t stab, const float4* __restrict__ Q, const uint32_t
 __global__ void __launch_bounds__(kAdjThreads)
 k_adj_r1(DevParams P, uint32_t stab, const float4* __restrict__ Q, const float* __restrict__ LAM,
          const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
-         const float4* __restrict__ DPb, const float4* __restrict__ PUb, float4* __restrict__ Pn,
-         float4* __restrict__ AB, float* __restrict__ Rb, double2* __restrict__ acc, uint32_t* __restrict__ flag) {
+         const float4* __restrict__ DPb, const float4* __restrict__ PUb, const float4* __restrict__ QX,
+         float4* __restrict__ Pn, float4* __restrict__ AB, float* __restrict__ Rb, float4* __restrict__ XF,
+         double2* __restrict__ acc, uint32_t* __restrict__ flag) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t s = t / kLanes, lane = t % kLanes;
   if (s >= P.n) return;
@@ -299,13 +340,36 @@ k_adj_r1(DevParams P, uint32_t stab, const float4* __restrict__ Q, const float* 
   const float3 dpi = f3(DPb[s]), pui = f3(PUb[s]);
   const CellFrame f = make_frame(P, key);
   float A = 0.0f, B = 0.0f, lb = 0.0f;
-  float3 a = make_float3(0.f, 0.f, 0.f), dsum = a, pn = a;
+  float3 a = make_float3(0.f, 0.f, 0.f), dsum = a, pn = a, xf = a;
+  // phases: density terms between fluid particles only; friction (QX != nullptr)
+  // on contacts with a solid, from the displacements p* - x of the iteration
+  const uint32_t* __restrict__ ph = P.phs;
+  const bool si = ph && ph[s];
+  float3 di = make_float3(0.f, 0.f, 0.f);
+  if (QX) di = f3(qi) - f3(QX[s]);
   const uint32_t cnt = adj_visit(
       P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool contact) {
         const float4 qj = Q[j];
         const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
         const float r2 = dot(r, r);
         const AdjPair q = adj_pair(P, r2);
+        const bool touch = stab && contact && q.len < P.diameter && q.len > 0.00001f;
+        if (touch) {
+          const float3 pd = pui - f3(PUb[j]);
+          const float phi = (P.diameter - q.len) * q.ir;
+          const float rr = dot(r, pd) * P.diameter * (q.ir * q.ir * q.ir);
+          const float kc = -P.coll_scale;
+          pn += kc * (phi * pd - rr * r);
+        }
+        const bool sj = ph && ph[j];
+        if (QX && touch && (si || sj)) {
+          const float3 u = di - (f3(qj) - f3(QX[j]));
+          float3 gr, gu;
+          friction_adj(P, r, q.len, q.ir, u, dpi - f3(DPb[j]), gr, gu);
+          pn += gr + gu;
+          xf = xf - gu;
+        }
+        if (si || sj) return;
         const float3 G = q.gs * r;
         A += q.W;
         a += G;
@@ -317,13 +381,6 @@ k_adj_r1(DevParams P, uint32_t stab, const float4* __restrict__ Q, const float* 
         dsum += S * G;
         pn += (S * P.inv_rho0) * hmul(q, r, dd);
         pn += (dscorr_of(P, q.W) * gd * P.inv_rho0 * q.wgs) * r;
-        if (stab && contact && q.len < P.diameter && q.len > 0.00001f) {
-          const float3 pd = pui - f3(PUb[j]);
-          const float phi = (P.diameter - q.len) * q.ir;
-          const float rr = dot(r, pd) * P.diameter * (q.ir * q.ir * q.ir);
-          const float kc = -P.coll_scale;
-          pn += kc * (phi * pd - rr * r);
-        }
       });
   A = lane_sum(A);
   B = lane_sum(B);
@@ -331,7 +388,9 @@ k_adj_r1(DevParams P, uint32_t stab, const float4* __restrict__ Q, const float* 
   a = lane_sum(a);
   dsum = lane_sum(dsum);
   pn = dpi + lane_sum(pn);
+  if (QX) xf = lane_sum(xf);
   if (lane) return;
+  if (QX) XF[s] = make_float4(xf.x, xf.y, xf.z, 0.0f);
   if (cnt >= P.cap) flag[0] = 1u;
   lb *= P.inv_rho0;
   const float rho = A + P.w0;
@@ -357,8 +416,8 @@ k_adj_r1(DevParams P, uint32_t stab, const float4* __restrict__ Q, const float* 
 __global__ void __launch_bounds__(kAdjThreads)
 k_adj_r2(DevParams P, uint32_t stab, uint32_t last, const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
          const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce, const float4* __restrict__ AB,
-         const float* __restrict__ Rb, const float4* __restrict__ Pn, const float4* __restrict__ Xb,
-         float4* __restrict__ DPb, float4* __restrict__ PUb, float4* __restrict__ Pb) {
+         const float* __restrict__ Rb, const float4* __restrict__ Pn, const float4* __restrict__ XF,
+         float4* __restrict__ Xb, float4* __restrict__ DPb, float4* __restrict__ PUb, float4* __restrict__ Pb) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t s = t / kLanes, lane = t % kLanes;
   if (s >= P.n) return;
@@ -368,7 +427,10 @@ k_adj_r2(DevParams P, uint32_t stab, uint32_t last, const float4* __restrict__ Q
   const float rbi = Rb[s];
   const CellFrame f = make_frame(P, key);
   float3 pn = make_float3(0.f, 0.f, 0.f);
+  const uint32_t* __restrict__ ph = P.phs;  // phases: fluid pairs only
+  const bool si = ph && ph[s];
   adj_visit(P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
+                     if (si || (ph && ph[j])) return;
                      const float4 qj = Q[j];
                      const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
                      const AdjPair q = adj_pair(P, dot(r, r));
@@ -378,13 +440,19 @@ k_adj_r2(DevParams P, uint32_t stab, uint32_t last, const float4* __restrict__ Q
                    });
   pn = f3(Pn[s]) + lane_sum(pn);
   if (lane) return;
+  // friction reads x of this iteration: its adjoint joins x's before it
+  float4 xv = Xb[s];
+  if (XF) {
+    const float4 xf = XF[s];
+    xv = make_float4(xv.x + xf.x, xv.y + xf.y, xv.z + xf.z, 0.0f);
+    Xb[s] = xv;
+  }
   if (last) {
     Pb[s] = make_float4(pn.x, pn.y, pn.z, 0.0f);
     return;
   }
   const float4 d = masked(pn, clamp_mask(P, key, qi));
   DPb[s] = d;
-  const float4 xv = Xb[s];
   PUb[s] = stab ? make_float4(d.x + xv.x, d.y + xv.y, d.z + xv.z, 0.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
@@ -462,14 +530,17 @@ k_adj_reduce(uint32_t n, const double2* __restrict__ acc, double* __restrict__ o
 
 }  // namespace
 
-hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P, uint32_t iters, uint32_t stab,
+hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P0, uint32_t iters, uint32_t stab,
                                const AdjTape& T, const AdjWork& W) {
-  const uint32_t n = P.n;
+  const uint32_t n = P0.n;
   if (n == 0) return hipSuccess;
   if (iters == 0) return hipErrorInvalidValue;  // the host rejects steps without solver iterations
   const dim3 grid((n * kLanes + kAdjThreads - 1) / kAdjThreads), block(kAdjThreads);
   const dim3 grid1((n + kAdjThreads - 1) / kAdjThreads);
   const float4* Qf = T.Q + (size_t)iters * n;
+  DevParams P = P0;  // the phases of this step (P.phs reads the record)
+  P.phs = T.PH;
+  const bool fric = T.QX != nullptr;
   k_adj_a<<<grid, block, 0, st>>>(P, Qf, T.keys, T.cs, T.ce, T.U, T.OM, W.GV, W.GF, W.Ub, W.Xp, W.Eb, W.Ob, W.acc,
                                   W.flag);
   k_adj_b<<<grid, block, 0, st>>>(P, Qf, T.keys, T.cs, T.ce, T.OM, W.Eb, W.Ob, W.Xp);
@@ -477,10 +548,11 @@ hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P, uint32_t iter
                                   W.PUb, W.acc);
   for (uint32_t it = iters; it-- > 0;) {
     const float4* Q = T.Q + (size_t)it * n;
-    k_adj_r1<<<grid, block, 0, st>>>(P, stab, Q, T.LAM + (size_t)it * n, T.keys, T.cs, T.ce, W.DPb, W.PUb, W.Pn,
-                                     W.AB, W.Rb, W.acc, W.flag);
-    k_adj_r2<<<grid, block, 0, st>>>(P, stab, it == 0 ? 1u : 0u, Q, T.keys, T.cs, T.ce, W.AB, W.Rb, W.Pn, W.Xb,
-                                     W.DPb, W.PUb, W.Pb);
+    const float4* QX = fric ? T.QX + (size_t)it * n : nullptr;
+    k_adj_r1<<<grid, block, 0, st>>>(P, stab, Q, T.LAM + (size_t)it * n, T.keys, T.cs, T.ce, W.DPb, W.PUb, QX, W.Pn,
+                                     W.AB, W.Rb, fric ? W.XF : nullptr, W.acc, W.flag);
+    k_adj_r2<<<grid, block, 0, st>>>(P, stab, it == 0 ? 1u : 0u, Q, T.keys, T.cs, T.ce, W.AB, W.Rb, W.Pn,
+                                     fric ? W.XF : nullptr, W.Xb, W.DPb, W.PUb, W.Pb);
   }
   k_adj_predict<<<grid1, block, 0, st>>>(P, T.perm, T.Xin, T.Vin, T.Fin, W.Pb, W.Xb, W.GX, W.GV, W.GF);
   return hipGetLastError();

@@ -30,6 +30,8 @@ struct AdjTape {
   float4* Xin;     // n: the step's input state, order before the sort
   float4* Vin;
   float4* Fin;
+  uint32_t* PH;    // n: phases of the step (nullptr: all fluid)
+  float4* QX;      // iters x n: x (cell-local) at each iteration's start (friction only, else nullptr)
 };
 
 // Adjoint work buffers (n each, sorted order).
@@ -42,11 +44,12 @@ struct AdjWork {
   float4* Eb;   // eta adjoint
   float4* Ob;   // omega adjoint
   float4* Pb;   // predicted-position adjoint
-  float4* Xb;   // position (x) adjoint, constant through the iterations
+  float4* Xb;   // position (x) adjoint; through the iterations it changes only by friction
   float4* DPb;  // delta-p adjoint (clamp-masked)
   float4* PUb;  // push adjoint
   float4* AB;   // adjoint of a_i = sum_j grad W_ij (xyz) and of b_i = sum_j |grad W_ij|^2 (w)
   float4* Pn;   // partial predicted-position adjoint of an iteration
+  float4* XF;   // friction's x adjoint of an iteration (friction only)
   float* Rb;    // density adjoint
   double2* acc; // per slot: accumulated d L / d rho0 (x) and d L / d eps (y)
   uint32_t* flag;  // [0]: a neighbour list reached the cap

@@ -24,7 +24,10 @@ struct Fluid::AdjState {
   uint32_t n = 0, frames = 0, k = 0, sub = 0, iters = 0, stab = 0, nseg = 0, nrec = 0, table = 0;
   std::vector<void*> mem;
   std::vector<float4*> ckpt;    // per checkpoint: X | V | F (3n float4)
+  std::vector<uint32_t*> ckpt_ph;  // per checkpoint: the phases in the same order (phased contexts)
   float4* final_state = nullptr;  // X | P | V | C | F after the forward
+  uint32_t* final_ph = nullptr;   // phases after the forward (phased contexts)
+  bool phased = false, friction = false;
   std::vector<AdjTape> tape;    // records of one segment
   AdjWork W{};
   uint32_t* ids[2] = {nullptr, nullptr};
@@ -66,7 +69,6 @@ int Fluid::adj_check() {
       !p.active_compute_omegas || !p.active_compute_vorticity || !p.active_compute_viscosity)
     return fail(PBD_ERR_UNSUPPORTED, "adjoint: every Active.* stage but findNeighbours must be on");
   if (scene_table_) return fail(PBD_ERR_UNSUPPORTED, "adjoint: not available for batched scenes");
-  if (phased()) return fail(PBD_ERR_UNSUPPORTED, "adjoint: not available with solid particles or friction");
   if (n_ == 0) return fail(PBD_ERR_INVALID, "adjoint: no particles");
   return PBD_OK;
 }
@@ -95,23 +97,30 @@ int Fluid::adj_forward(uint32_t frames, uint32_t k_ckpt) {
   A.nrec = A.k * A.sub;
   A.table = B_.table_size;
   A.params = params_;
+  // phases: the records hold each step's sorted phases, and with friction x at
+  // every iteration's start; checkpoints keep the phases in the state's order
+  A.phased = phased();
+  A.friction = A.phased && params_.friction_static > 0.0f && A.stab > 0;
   const size_t n = n_;
   bool ok = true;
   A.ckpt.assign(A.nseg, nullptr);
-  for (uint32_t g = 0; g < A.nseg && ok; ++g) ok = A.alloc(&A.ckpt[g], 3 * n);
-  ok = ok && A.alloc(&A.final_state, 5 * n);
+  A.ckpt_ph.assign(A.nseg, nullptr);
+  for (uint32_t g = 0; g < A.nseg && ok; ++g) ok = A.alloc(&A.ckpt[g], 3 * n) && (!A.phased || A.alloc(&A.ckpt_ph[g], n));
+  ok = ok && A.alloc(&A.final_state, 5 * n) && (!A.phased || A.alloc(&A.final_ph, n));
   A.tape.assign(A.nrec, AdjTape{});
   for (uint32_t t = 0; t < A.nrec && ok; ++t) {
     AdjTape& T = A.tape[t];
     ok = A.alloc(&T.Q, (size_t)(A.iters + 1) * n) && A.alloc(&T.LAM, (size_t)A.iters * n) && A.alloc(&T.U, n) &&
          A.alloc(&T.OM, n) && A.alloc(&T.keys, n) && A.alloc(&T.perm, n) && A.alloc(&T.cs, A.table) &&
-         A.alloc(&T.ce, A.table) && A.alloc(&T.Xin, n) && A.alloc(&T.Vin, n) && A.alloc(&T.Fin, n);
+         A.alloc(&T.ce, A.table) && A.alloc(&T.Xin, n) && A.alloc(&T.Vin, n) && A.alloc(&T.Fin, n) &&
+         (!A.phased || A.alloc(&T.PH, n)) && (!A.friction || A.alloc(&T.QX, (size_t)A.iters * n));
   }
   AdjWork& W = A.W;
   ok = ok && A.alloc(&W.GX, n) && A.alloc(&W.GV, n) && A.alloc(&W.GF, n) && A.alloc(&W.Ub, n) && A.alloc(&W.Xp, n) &&
        A.alloc(&W.Eb, n) && A.alloc(&W.Ob, n) && A.alloc(&W.Pb, n) && A.alloc(&W.Xb, n) && A.alloc(&W.DPb, n) &&
        A.alloc(&W.PUb, n) && A.alloc(&W.AB, n) && A.alloc(&W.Pn, n) && A.alloc(&W.Rb, n) && A.alloc(&W.acc, n) &&
-       A.alloc(&W.flag, 1) && A.alloc(&A.ids[0], n) && A.alloc(&A.ids[1], n) && A.alloc(&A.red, 2);
+       A.alloc(&W.flag, 1) && A.alloc(&A.ids[0], n) && A.alloc(&A.ids[1], n) && A.alloc(&A.red, 2) &&
+       (!A.friction || A.alloc(&W.XF, n));
   if (!ok) {
     adj_release();
     return fail(PBD_ERR_NOMEM, "adjoint: device allocation of checkpoints and step records failed");
@@ -120,10 +129,12 @@ int Fluid::adj_forward(uint32_t frames, uint32_t k_ckpt) {
   std::iota(iota.begin(), iota.end(), 0u);
   if (hip_check(hipMemcpy(A.ids[0], iota.data(), n * 4, hipMemcpyHostToDevice), "adjoint ids")) return status_;
   A.ids_cur = 0;
-  auto save = [&](float4* dst) {
-    return tape_copy(dst, B_.X, n * 16) || tape_copy(dst + n, B_.V, n * 16) || tape_copy(dst + 2 * n, B_.F, n * 16);
+  auto save = [&](uint32_t g) {
+    float4* dst = A.ckpt[g];
+    return tape_copy(dst, B_.X, n * 16) || tape_copy(dst + n, B_.V, n * 16) || tape_copy(dst + 2 * n, B_.F, n * 16) ||
+           (A.phased && tape_copy(A.ckpt_ph[g], ph_, n * 4));
   };
-  if (save(A.ckpt[0])) return status_;
+  if (save(0)) return status_;
   if (int rc = ensure_graph()) return rc;
   // The last segment's steps write their records as they run (eagerly, the
   // same kernels as the graph), so the reverse pass starts without recomputing it.
@@ -142,12 +153,13 @@ int Fluid::adj_forward(uint32_t frames, uint32_t k_ckpt) {
         return status_;
       A.ids_cur = 1 - A.ids_cur;
     }
-    if ((f + 1) % A.k == 0 && f + 1 < frames && save(A.ckpt[(f + 1) / A.k])) return status_;
+    if ((f + 1) % A.k == 0 && f + 1 < frames && save((f + 1) / A.k)) return status_;
   }
   A.tail_recorded = true;
   float4* fs = A.final_state;
   if (tape_copy(fs, B_.X, n * 16) || tape_copy(fs + n, B_.P, n * 16) || tape_copy(fs + 2 * n, B_.V, n * 16) ||
-      tape_copy(fs + 3 * n, B_.C, n * 16) || tape_copy(fs + 4 * n, B_.F, n * 16))
+      tape_copy(fs + 3 * n, B_.C, n * 16) || tape_copy(fs + 4 * n, B_.F, n * 16) ||
+      (A.phased && tape_copy(A.final_ph, ph_, n * 4)))
     return status_;
   A.valid = true;
   return PBD_OK;
@@ -184,7 +196,8 @@ int Fluid::adj_backward(const float* gx, const float* gv, float* gv0, float* grh
     const uint32_t nstep = (std::min(A.frames, (g + 1) * A.k) - g * A.k) * A.sub;
     if (g + 1 < A.nseg || !A.tail_recorded) {
       const float4* c = A.ckpt[g];
-      if (tape_copy(B_.X, c, n * 16) || tape_copy(B_.V, c + n, n * 16) || tape_copy(B_.F, c + 2 * n, n * 16))
+      if (tape_copy(B_.X, c, n * 16) || tape_copy(B_.V, c + n, n * 16) || tape_copy(B_.F, c + 2 * n, n * 16) ||
+          (A.phased && tape_copy(ph_, A.ckpt_ph[g], n * 4)))
         return status_;
       for (uint32_t t = 0; t < nstep; ++t) {
         rec_ = &A.tape[t];
@@ -212,6 +225,7 @@ int Fluid::adj_backward(const float* gx, const float* gv, float* gv0, float* grh
   if (e == hipSuccess) e = hipMemcpyAsync(B_.V, fs + 2 * n, n * 16, hipMemcpyDeviceToDevice, stream_);
   if (e == hipSuccess) e = hipMemcpyAsync(B_.C, fs + 3 * n, n * 16, hipMemcpyDeviceToDevice, stream_);
   if (e == hipSuccess) e = hipMemcpyAsync(B_.F, fs + 4 * n, n * 16, hipMemcpyDeviceToDevice, stream_);
+  if (e == hipSuccess && A.phased) e = hipMemcpyAsync(ph_, A.final_ph, n * 4, hipMemcpyDeviceToDevice, stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
   if (hip_check(e, "adjoint: results")) return status_;
   if (grho0) *grho0 = (float)red[0];

@@ -374,6 +374,7 @@ int Fluid::enqueue_substep() {
     if (tape_copy(rec_->Q, B_.Q[0], n16) || tape_copy(rec_->keys, keys_sorted_, n4) ||
         tape_copy(rec_->perm, ids_sorted_, n4) || tape_copy(rec_->cs, B_.cstart, tb) || tape_copy(rec_->ce, B_.cend, tb))
       return status_;
+    if (rec_->PH && tape_copy(rec_->PH, phs_, n4)) return status_;  // the step's phases, sorted
   }
 
   // 6-7. solver loop (Fluid.cu:49-80); contacts/neighbours are walked inside the passes
@@ -390,6 +391,8 @@ int Fluid::enqueue_substep() {
   for (uint32_t it = 0; it < iters; ++it) {
     const bool last_it = it + 1 == iters;
     bool listed = false;  // the lambda pass left the neighbour lists of Q[cur]
+    // x at the iteration's start, for the reverse of the friction
+    if (rec_ && rec_->QX && tape_copy(rec_->QX + (size_t)it * P.n, B_.QX[cur], n16)) return status_;
     if (lambda_on) {
       if (tiles) {
         PBD_LAUNCH(ST_LAMBDA, launch_tile_lambda(st, bits, tgrid, tstride, P, TL, B_, B_.Q[cur], keys, !counted));
