@@ -204,9 +204,100 @@ __device__ __forceinline__ uint32_t adj_visit(const DevParams& P, uint32_t s, ui
   return cnt;
 }
 
+// Neighbour lists of the reverse passes.  For one set of positions, each lane
+// of a particle keeps the candidates of its share of the 27-cell stencil that
+// lie within the list radius (every pair term is zero beyond it), at most
+// kAdjListMax of them, as j | (27-cell offset << 27).  A particle with a fuller
+// lane is marked kAdjScan on all its lanes, which then scan the stencil.  The
+// passes walk the lists instead of the ~216 candidates, so a wave executes a
+// pair body about 8 instead of 27 times per lane.
+constexpr uint32_t kAdjListMax = 8;
+constexpr uint32_t kAdjScan = 0xFFFFFFFFu;
+struct AdjList {
+  uint32_t* e;    // [(k n + s) kLanes + lane], k < kAdjListMax (nullptr: no lists, scan)
+  uint32_t* cnt;  // [s kLanes + lane]
+};
+
+__device__ __forceinline__ bool lane_any(bool v) {
+  uint32_t x = v ? 1u : 0u;
+#pragma unroll
+  for (uint32_t o = kLanes / 2; o > 0; o >>= 1) x |= (uint32_t)__shfl_xor((int)x, (int)o);
+  return x != 0u;
+}
+
+__global__ void __launch_bounds__(kAdjThreads)
+k_adj_list(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
+           const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce, AdjList L, uint32_t* __restrict__ flag) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t s = t / kLanes, lane = t % kLanes;
+  if (s >= P.n) return;
+  const uint32_t n = P.n;
+  const float4 qi = Q[s];
+  const CellFrame f = make_frame(P, keys[s]);
+  uint32_t c = 0, cnt = 0;
+  for (int dx = -1; dx <= 1; ++dx) {
+    const uint32_t nx = f.ix + (uint32_t)dx;
+    if (nx >= P.dims[0]) continue;
+    const uint32_t kx = dilated_add(f.kx, dx, kMortonX, 2);
+    const float bx = qi.x + (f.cx - cell_corner(P.origin[0], P.cell, nx));
+    for (int dy = -1; dy <= 1; ++dy) {
+      const uint32_t ny = f.iy + (uint32_t)dy;
+      if (ny >= P.dims[1]) continue;
+      const uint32_t ky = dilated_add(f.ky, dy, kMortonY, 1);
+      const float by = qi.y + (f.cy - cell_corner(P.origin[1], P.cell, ny));
+      for (int dz = -1; dz <= 1; ++dz) {
+        const uint32_t nz = f.iz + (uint32_t)dz;
+        if (nz >= P.dims[2]) continue;
+        const uint32_t kz = dilated_add(f.kz, dz, kMortonZ, 0);
+        const float bz = qi.z + (f.cz - cell_corner(P.origin[2], P.cell, nz));
+        const uint32_t k = kx | ky | kz;
+        const uint32_t st = cs[k];
+        if (st == 0xFFFFFFFFu) continue;
+        const uint32_t en = ce[k];
+        cnt += en - st - ((s >= st && s < en) ? 1u : 0u);
+        const uint32_t off = (uint32_t)((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)) << 27;
+        for (uint32_t j = st + ((lane - st) & (kLanes - 1)); j < en; j += kLanes) {
+          if (j == s) continue;
+          const float4 qj = Q[j];
+          const float rx = bx - qj.x, ry = by - qj.y, rz = bz - qj.z;
+          if (rx * rx + ry * ry + rz * rz < P.list_r2) {
+            if (c < kAdjListMax) L.e[((size_t)c * n + s) * kLanes + lane] = j | off;
+            ++c;
+          }
+        }
+      }
+    }
+  }
+  const bool over = lane_any(c > kAdjListMax);
+  L.cnt[(size_t)s * kLanes + lane] = over ? kAdjScan : c;
+  if (lane == 0 && cnt >= P.cap) flag[0] = 1u;
+}
+
+// adj_visit over the lane's list when there is one (27-cell stencil: every
+// entry is a contact); returns the particle's neighbour count only when it scans.
+template <class Body>
+__device__ __forceinline__ uint32_t adj_walk(const DevParams& P, const AdjList& L, uint32_t s, uint32_t lane,
+                                             const CellFrame& f, float qx, float qy, float qz,
+                                             const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+                                             Body body) {
+  const uint32_t c = L.e ? L.cnt[(size_t)s * kLanes + lane] : kAdjScan;
+  if (c == kAdjScan) return adj_visit(P, s, lane, f, qx, qy, qz, cs, ce, body);
+  const uint32_t n = P.n;
+  for (uint32_t k = 0; k < c; ++k) {
+    const uint32_t e = L.e[((size_t)k * n + s) * kLanes + lane];
+    const uint32_t j = e & 0x07FFFFFFu, off = e >> 27;
+    const int dx = (int)(off / 9u) - 1, dy = (int)(off / 3u % 3u) - 1, dz = (int)(off % 3u) - 1;
+    const float bx = qx + (f.cx - cell_corner(P.origin[0], P.cell, f.ix + (uint32_t)dx));
+    const float by = qy + (f.cy - cell_corner(P.origin[1], P.cell, f.iy + (uint32_t)dy));
+    const float bz = qz + (f.cz - cell_corner(P.origin[2], P.cell, f.iz + (uint32_t)dz));
+    body(j, bx, by, bz, true);
+  }
+  return 0u;
+}
+
 // ---------------------------------------------------------------- A
 __global__ void __launch_bounds__(kAdjThreads)
-k_adj_a(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
+k_adj_a(DevParams P, AdjList L, const float4* __restrict__ Q, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
         const uint32_t* __restrict__ ce, const float4* __restrict__ U, const float4* __restrict__ OM,
         const float4* __restrict__ GV, const float4* __restrict__ GF, float4* __restrict__ Ub,
         float4* __restrict__ Xp, float4* __restrict__ Eb, float4* __restrict__ Ob, double2* __restrict__ acc,
@@ -221,7 +312,7 @@ k_adj_a(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ 
   const uint32_t* __restrict__ ph = P.phs;  // phases: fluid pairs only
   const bool si = ph && ph[s];
   float3 ub = make_float3(0.f, 0.f, 0.f), xb = ub, eta = ub, xs = ub;
-  const uint32_t cnt = adj_visit(P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
+  const uint32_t cnt = adj_walk(P, L, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
         if (si || (ph && ph[j])) return;
         const float4 qj = Q[j];
         const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
@@ -253,7 +344,7 @@ k_adj_a(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ 
 
 // ---------------------------------------------------------------- B
 __global__ void __launch_bounds__(kAdjThreads)
-k_adj_b(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
+k_adj_b(DevParams P, AdjList L, const float4* __restrict__ Q, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
         const uint32_t* __restrict__ ce, const float4* __restrict__ OM, const float4* __restrict__ Eb,
         float4* __restrict__ Ob, float4* __restrict__ Xp) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -266,7 +357,7 @@ k_adj_b(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ 
   float3 xb = make_float3(0.f, 0.f, 0.f);
   const uint32_t* __restrict__ ph = P.phs;  // phases: fluid pairs only
   const bool si = ph && ph[s];
-  adj_visit(P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
+  adj_walk(P, L, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
                      if (si || (ph && ph[j])) return;
                      const float4 qj = Q[j];
                      const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
@@ -287,7 +378,7 @@ k_adj_b(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ 
 
 // ---------------------------------------------------------------- C
 __global__ void __launch_bounds__(kAdjThreads)
-k_adj_c(DevParams P, uint32_t stab, const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
+k_adj_c(DevParams P, AdjList L, uint32_t stab, const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
         const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce, const float4* __restrict__ U,
         const float4* __restrict__ OM, const float4* __restrict__ Ob, const float4* __restrict__ Ub,
         const float4* __restrict__ Xp, const float4* __restrict__ GX, float4* __restrict__ Xb,
@@ -302,7 +393,7 @@ k_adj_c(DevParams P, uint32_t stab, const float4* __restrict__ Q, const uint32_t
   float3 ub = make_float3(0.f, 0.f, 0.f), xb = ub;
   const uint32_t* __restrict__ ph = P.phs;  // phases: fluid pairs only
   const bool si = ph && ph[s];
-  adj_visit(P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
+  adj_walk(P, L, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
                      if (si || (ph && ph[j])) return;
                      const float4 qj = Q[j];
                      const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
@@ -326,7 +417,7 @@ k_adj_c(DevParams P, uint32_t stab, const float4* __restrict__ Q, const uint32_t
 
 // ---------------------------------------------------------------- R1
 __global__ void __launch_bounds__(kAdjThreads)
-k_adj_r1(DevParams P, uint32_t stab, const float4* __restrict__ Q, const float* __restrict__ LAM,
+k_adj_r1(DevParams P, AdjList L, uint32_t stab, const float4* __restrict__ Q, const float* __restrict__ LAM,
          const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
          const float4* __restrict__ DPb, const float4* __restrict__ PUb, const float4* __restrict__ QX,
          float4* __restrict__ Pn, float4* __restrict__ AB, float* __restrict__ Rb, float4* __restrict__ XF,
@@ -347,8 +438,8 @@ k_adj_r1(DevParams P, uint32_t stab, const float4* __restrict__ Q, const float* 
   const bool si = ph && ph[s];
   float3 di = make_float3(0.f, 0.f, 0.f);
   if (QX) di = f3(qi) - f3(QX[s]);
-  const uint32_t cnt = adj_visit(
-      P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool contact) {
+  const uint32_t cnt = adj_walk(
+      P, L, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool contact) {
         const float4 qj = Q[j];
         const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
         const float r2 = dot(r, r);
@@ -414,7 +505,7 @@ k_adj_r1(DevParams P, uint32_t stab, const float4* __restrict__ Q, const float* 
 // last: the first iteration (it = 0): Pb <- the adjoint of its input
 // positions; otherwise DPb / PUb <- the masked adjoint for iteration it - 1.
 __global__ void __launch_bounds__(kAdjThreads)
-k_adj_r2(DevParams P, uint32_t stab, uint32_t last, const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
+k_adj_r2(DevParams P, AdjList L, uint32_t stab, uint32_t last, const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
          const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce, const float4* __restrict__ AB,
          const float* __restrict__ Rb, const float4* __restrict__ Pn, const float4* __restrict__ XF,
          float4* __restrict__ Xb, float4* __restrict__ DPb, float4* __restrict__ PUb, float4* __restrict__ Pb) {
@@ -429,7 +520,7 @@ k_adj_r2(DevParams P, uint32_t stab, uint32_t last, const float4* __restrict__ Q
   float3 pn = make_float3(0.f, 0.f, 0.f);
   const uint32_t* __restrict__ ph = P.phs;  // phases: fluid pairs only
   const bool si = ph && ph[s];
-  adj_visit(P, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
+  adj_walk(P, L, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
                      if (si || (ph && ph[j])) return;
                      const float4 qj = Q[j];
                      const float3 r = make_float3(bx - qj.x, by - qj.y, bz - qj.z);
@@ -541,22 +632,30 @@ hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P0, uint32_t ite
   DevParams P = P0;  // the phases of this step (P.phs reads the record)
   P.phs = T.PH;
   const bool fric = T.QX != nullptr;
-  k_adj_a<<<grid, block, 0, st>>>(P, Qf, T.keys, T.cs, T.ce, T.U, T.OM, W.GV, W.GF, W.Ub, W.Xp, W.Eb, W.Ob, W.acc,
+  // neighbour lists for the 27-cell stencil (shells of a wider stencil scan)
+  const bool lists = W.LE && n >= kAdjListMinParticles && n < (1u << 27) && !((P.gates & G_FIND_NEIGHBOURS) && P.kw > 1);
+  const AdjList L{lists ? W.LE : nullptr, lists ? W.LC : nullptr};
+  if (lists) k_adj_list<<<grid, block, 0, st>>>(P, Qf, T.keys, T.cs, T.ce, L, W.flag);
+  k_adj_a<<<grid, block, 0, st>>>(P, L, Qf, T.keys, T.cs, T.ce, T.U, T.OM, W.GV, W.GF, W.Ub, W.Xp, W.Eb, W.Ob, W.acc,
                                   W.flag);
-  k_adj_b<<<grid, block, 0, st>>>(P, Qf, T.keys, T.cs, T.ce, T.OM, W.Eb, W.Ob, W.Xp);
-  k_adj_c<<<grid, block, 0, st>>>(P, stab, Qf, T.keys, T.cs, T.ce, T.U, T.OM, W.Ob, W.Ub, W.Xp, W.GX, W.Xb, W.DPb,
+  k_adj_b<<<grid, block, 0, st>>>(P, L, Qf, T.keys, T.cs, T.ce, T.OM, W.Eb, W.Ob, W.Xp);
+  k_adj_c<<<grid, block, 0, st>>>(P, L, stab, Qf, T.keys, T.cs, T.ce, T.U, T.OM, W.Ob, W.Ub, W.Xp, W.GX, W.Xb, W.DPb,
                                   W.PUb, W.acc);
   for (uint32_t it = iters; it-- > 0;) {
     const float4* Q = T.Q + (size_t)it * n;
     const float4* QX = fric ? T.QX + (size_t)it * n : nullptr;
-    k_adj_r1<<<grid, block, 0, st>>>(P, stab, Q, T.LAM + (size_t)it * n, T.keys, T.cs, T.ce, W.DPb, W.PUb, QX, W.Pn,
+    if (lists) k_adj_list<<<grid, block, 0, st>>>(P, Q, T.keys, T.cs, T.ce, L, W.flag);
+    k_adj_r1<<<grid, block, 0, st>>>(P, L, stab, Q, T.LAM + (size_t)it * n, T.keys, T.cs, T.ce, W.DPb, W.PUb, QX, W.Pn,
                                      W.AB, W.Rb, fric ? W.XF : nullptr, W.acc, W.flag);
-    k_adj_r2<<<grid, block, 0, st>>>(P, stab, it == 0 ? 1u : 0u, Q, T.keys, T.cs, T.ce, W.AB, W.Rb, W.Pn,
+    k_adj_r2<<<grid, block, 0, st>>>(P, L, stab, it == 0 ? 1u : 0u, Q, T.keys, T.cs, T.ce, W.AB, W.Rb, W.Pn,
                                      fric ? W.XF : nullptr, W.Xb, W.DPb, W.PUb, W.Pb);
   }
   k_adj_predict<<<grid1, block, 0, st>>>(P, T.perm, T.Xin, T.Vin, T.Fin, W.Pb, W.Xb, W.GX, W.GV, W.GF);
   return hipGetLastError();
 }
+
+size_t adjoint_list_entries(uint32_t n) { return (size_t)n * kLanes * kAdjListMax; }
+size_t adjoint_list_counts(uint32_t n) { return (size_t)n * kLanes; }
 
 hipError_t launch_gather_ids(hipStream_t st, uint32_t n, const uint32_t* perm, const uint32_t* ids_in,
                              uint32_t* ids_out) {

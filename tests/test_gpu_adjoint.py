@@ -253,6 +253,55 @@ def test_gpu_adjoint_with_phases_matches_oracle_adjoint(F, O, tile_bits):
         assert _rel(g[k], go[k]) < 1e-3
 
 
+def test_gpu_adjoint_neighbour_lists_262k(F, O):
+    """From 2^18 particles on, the reverse passes walk neighbour lists built per
+    position set instead of scanning the stencil: a 64^3 lattice (the 1M
+    dam-break parameters) with 30 % solids and friction, one frame, against
+    the FP64 oracle's reverse mode."""
+    p = F.load_params(F.config_path("dam_break_1m"))
+    p.box_width = p.box_height = p.box_depth = 64
+    p.friction_static = 0.5
+    p.friction_kinetic = 0.4
+    p.box_solid_fraction = 0.3
+    p.box_solid_radius = 0.1
+    pos, _, _ = F.generate_box(p)
+    ph = F.capi.generate_phases(p)
+    x0 = pos[:, :3].astype(np.float32)
+    n = x0.shape[0]
+    assert n == 1 << 18
+    rng = np.random.default_rng(5)
+    v0 = rng.normal(0.0, 0.1, x0.shape).astype(np.float32)
+    nsub = p.substeps
+    flush(f"neighbour-list reverse: n={n} solid {ph.mean():.2f} steps={nsub}")
+    x64, v64 = x0.astype(np.float64), v0.astype(np.float64)
+    X, _, _, og = O.forward64(p, x64, v64, nsub=nsub, phase=ph)
+    target = X[np.argsort(og)] + rng.normal(0.0, 0.01, x0.shape)
+    g, ids, xT = _gpu_grad_phased(F, p, x0, v0, ph, 1, 1, target)
+    # a few of the 262K particles sit within rounding of a friction branch switch
+    ex = np.abs(X[np.argsort(og)] - xT[np.argsort(ids)]).max(axis=1)
+    flush(f"  forward: max {ex.max():.2e}, 99.9th pct {np.quantile(ex, 0.999):.2e}")
+    assert np.quantile(ex, 0.999) < 1e-4 and ex.max() < 1e-3
+    gxo = 2.0 * (X - target[og]) / n
+    go = O.gradient64(p, x64, v64, gxo, np.zeros_like(gxo), nsub=nsub, phase=ph)
+    # The gradient error is a heavy tail: contacts whose FP32 and FP64 forwards
+    # take different sides of a branch (contact distance, static against
+    # kinetic friction) carry O(1) relative error on their few particles, so
+    # the bound of 1e-3 holds with the worst 0.3 % excluded, and the whole set
+    # stays within 1e-2.  A list that dropped or misplaced neighbours would
+    # move every particle's gradient.
+    drop = int(0.003 * n)
+    for k in ("v0", "x0"):
+        d = g[k] - go[k]
+        keep = np.argsort(np.linalg.norm(d, axis=1))[: n - drop]
+        err = np.linalg.norm(d) / np.linalg.norm(go[k])
+        trim = np.linalg.norm(d[keep]) / np.linalg.norm(go[k][keep])
+        flush(f"  {k}: relative L2 error {err:.2e}, without the worst {drop} particles {trim:.2e}")
+        assert trim < 1e-3 and err < 1e-2
+    for k in ("rho0", "eps"):
+        flush(f"  {k}: gpu {g[k]:.6e} oracle {go[k]:.6e}")
+        assert _rel(g[k], go[k]) < 1e-2
+
+
 def test_gpu_adjoint_with_phases_checkpoints_are_exact(F, O):
     """Checkpoints keep the phases in the state's order: recomputed segments
     replay the phased forward bit for bit."""
