@@ -110,7 +110,7 @@ int Fluid::init(const pbd_params& p, uint32_t capacity) {
   rc |= dev_alloc(allocations_, &phs_, c);
   sort_.n_max = cap_;
   sort_.tiles_max = (cap_ + kSortTile - 1) / kSortTile;
-  rc |= dev_alloc(allocations_, &sort_.hist, 4 * 256);
+  rc |= dev_alloc(allocations_, &sort_.hist, kSortHistWords);
   rc |= dev_alloc(allocations_, &sort_.tile_ctr, 4);
   rc |= dev_alloc(allocations_, &sort_.status, (size_t)4 * sort_.tiles_max * 256);
   rc |= dev_alloc(allocations_, &sort_.err, 1);

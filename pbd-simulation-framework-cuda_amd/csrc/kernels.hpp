@@ -127,7 +127,7 @@ struct SortPlan {
   uint32_t passes;      // 8-bit digit passes (1..4)
   uint32_t tiles_max;   // tiles for n_max
   // device scratch
-  uint32_t* hist;       // [4][256] digit counts then exclusive offsets
+  uint32_t* hist;       // [4][256] exclusive digit offsets, then kSortHistCopies x [4][256] partial counts
   uint32_t* tile_ctr;   // [4] tile tickets
   uint32_t* status;     // [4][tiles_max][256] look-back words
   uint32_t* err;        // look-back spin-limit flag
@@ -143,6 +143,10 @@ struct SortPlan {
 constexpr uint32_t kSortThreads = 256;
 constexpr uint32_t kSortItems = PBD_SORT_ITEMS;  // keys per thread of a onesweep tile
 constexpr uint32_t kSortTile = kSortThreads * kSortItems;
+// copies of the digit histogram that the counting workgroups add into (by
+// workgroup index), so that no more than 1/32 of them add to the same word
+constexpr uint32_t kSortHistCopies = 32;
+constexpr uint32_t kSortHistWords = (1u + kSortHistCopies) * 4u * 256u;
 
 // Number of 8-bit passes covering keys < key_bound.
 uint32_t sort_passes_for(uint32_t key_bound);

@@ -43,45 +43,89 @@ __global__ void k_sort_prepare(uint32_t* __restrict__ hist, uint32_t* __restrict
                                uint32_t* __restrict__ status, uint32_t status_words) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = gid; i < 4u * 256u; i += stride) hist[i] = 0u;
+  for (uint32_t i = gid; i < kSortHistWords; i += stride) hist[i] = 0u;
   if (gid < 4u) tile_ctr[gid] = 0u;
   for (uint32_t i = gid; i < status_words; i += stride) status[i] = 0u;
 }
 
+// Digit counts of every pass, over tiles of 16 keys per thread whose loads are
+// all in flight at once.  Sorted-order keys change slowly, so the lanes of a
+// wave mostly share a digit: one lane then adds the wave's count, and otherwise
+// the lanes with equal digits are found by ballots and one of them adds theirs,
+// instead of up to 64 same-address LDS atomics.  Workgroup b adds its counts to
+// copy b mod `copies` of the histogram, so that the same-address chains of
+// global atomics stay short.
+constexpr uint32_t kHistItems = 16;
 __global__ __launch_bounds__(256) void k_sort_hist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t passes,
-                                                   uint32_t* __restrict__ hist) {
+                                                   uint32_t copies, uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[4][256];
   for (uint32_t i = threadIdx.x; i < 4u * 256u; i += blockDim.x) (&h[0][0])[i] = 0u;
   __syncthreads();
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint32_t k = keys[i];
-    for (uint32_t p = 0; p < passes; ++p) atomicAdd(&h[p][(k >> (8u * p)) & 255u], 1u);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const uint32_t tile = 256u * kHistItems;
+  // the loop bound is uniform over the workgroup, so every lane takes part in the ballots
+  for (uint32_t t0 = blockIdx.x * tile; t0 < n; t0 += gridDim.x * tile) {
+    uint32_t k[kHistItems];
+#pragma unroll
+    for (uint32_t r = 0; r < kHistItems; ++r) {
+      const uint32_t i = t0 + r * 256u + threadIdx.x;
+      k[r] = i < n ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kHistItems; ++r) {
+      const bool valid = t0 + r * 256u + threadIdx.x < n;
+      const uint64_t live = __ballot(valid);
+      if (live == 0ull) continue;  // the valid lanes are a prefix: lane 0 is live
+      for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t d = (k[r] >> (8u * p)) & 255u;
+        const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+        if (__ballot(valid && d == d0) == live) {
+          if (lane == 0u) atomicAdd(&h[p][d0], (uint32_t)__popcll(live));
+          continue;
+        }
+        uint64_t m = live;
+#pragma unroll
+        for (uint32_t b = 0; b < 8u; ++b) {
+          const bool bit = (d >> b) & 1u;
+          const uint64_t bb = __ballot(bit);
+          m &= bit ? bb : ~bb;
+        }
+        if (valid && (m & lt) == 0ull) atomicAdd(&h[p][d], (uint32_t)__popcll(m));
+      }
+    }
   }
   __syncthreads();
+  uint32_t* part = hist + 1024u * (1u + blockIdx.x % copies);
   for (uint32_t p = 0; p < passes; ++p) {
     const uint32_t c = h[p][threadIdx.x];
-    if (c) atomicAdd(&hist[p * 256u + threadIdx.x], c);
+    if (c) atomicAdd(&part[p * 256u + threadIdx.x], c);
   }
 }
 
-// Exclusive scan of each pass's 256 digit counts (one workgroup of 256).
-__global__ __launch_bounds__(256) void k_sort_scan(uint32_t* __restrict__ hist, uint32_t passes) {
-  __shared__ uint32_t s[256];
-  const uint32_t t = threadIdx.x;
-  for (uint32_t p = 0; p < passes; ++p) {
-    const uint32_t v = hist[p * 256u + t];
-    s[t] = v;
+// Exclusive scan of each pass's 256 digit counts, summed over the histogram's
+// copies: one workgroup, a thread per (pass, digit), every copy's load in
+// flight at once.
+__global__ __launch_bounds__(1024) void k_sort_scan(uint32_t* __restrict__ hist, uint32_t passes, uint32_t copies) {
+  __shared__ uint32_t s[4][256];
+  const uint32_t t = threadIdx.x & 255u, p = threadIdx.x >> 8;
+  uint32_t v = 0;
+  if (p < passes) {
+    uint32_t a[kSortHistCopies];
+#pragma unroll
+    for (uint32_t c = 0; c < kSortHistCopies; ++c) a[c] = c < copies ? hist[1024u * (1u + c) + p * 256u + t] : 0u;
+#pragma unroll
+    for (uint32_t c = 0; c < kSortHistCopies; ++c) v += a[c];
+  }
+  s[p][t] = v;
+  __syncthreads();
+  for (uint32_t o = 1; o < 256u; o <<= 1) {
+    const uint32_t add = t >= o ? s[p][t - o] : 0u;
     __syncthreads();
-    for (uint32_t o = 1; o < 256u; o <<= 1) {
-      const uint32_t add = t >= o ? s[t - o] : 0u;
-      __syncthreads();
-      s[t] += add;
-      __syncthreads();
-    }
-    hist[p * 256u + t] = s[t] - v;
+    s[p][t] += add;
     __syncthreads();
   }
+  if (p < passes) hist[p * 256u + t] = s[p][t] - v;
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint32_t* __restrict__ kin,
@@ -229,6 +273,7 @@ hipError_t launch_sort(hipStream_t st, const SortPlan& plan, uint32_t n, const u
   const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
   const uint32_t passes = plan.passes;
   const uint32_t status_words = passes * tiles * 256u;
+  uint32_t copies = 1;
   {
     uint32_t b = (status_words + 255u) / 256u;
     if (b < 4u) b = 4u;
@@ -237,11 +282,14 @@ hipError_t launch_sort(hipStream_t st, const SortPlan& plan, uint32_t n, const u
                        status_words);
   }
   {
-    uint32_t b = (n + 256u * 16u - 1u) / (256u * 16u);
+    uint32_t b = (n + 256u * kHistItems - 1u) / (256u * kHistItems);
     if (b > 1024u) b = 1024u;
-    hipLaunchKernelGGL(k_sort_hist, dim3(b), dim3(256), 0, st, keys_in, n, passes, plan.hist);
+    copies = b / 32u;
+    if (copies < 1u) copies = 1u;
+    if (copies > kSortHistCopies) copies = kSortHistCopies;
+    hipLaunchKernelGGL(k_sort_hist, dim3(b), dim3(256), 0, st, keys_in, n, passes, copies, plan.hist);
   }
-  hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(256), 0, st, plan.hist, passes);
+  hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, plan.hist, passes, copies);
   const uint32_t* kin = keys_in;
   const uint32_t* vin = nullptr;
   for (uint32_t p = 0; p < passes; ++p) {
