@@ -26,6 +26,10 @@ constexpr uint32_t kFlagPre = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 24;
 constexpr uint32_t kWaves = kSortThreads / 64;
+#ifndef PBD_SORT_LOOKBACK
+#define PBD_SORT_LOOKBACK 4
+#endif
+constexpr uint32_t kLookBack = PBD_SORT_LOOKBACK;  // predecessor words read per look-back round trip
 }  // namespace
 
 uint32_t sort_passes_for(uint32_t key_bound) {
@@ -209,25 +213,42 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint32_t* __res
   }
   if (lane == 63u) s_wsum[wave] = x;
 
-  // Decoupled look-back for this digit.
+  // Decoupled look-back for this digit, kLookBack predecessors per round trip:
+  // their words are taken in order up to the first one not yet published (read
+  // again in the next round) or the first inclusive prefix (the end).
   uint32_t excl = 0;
   if (tile > 0) {
     int t = (int)tile - 1;
     uint32_t spins = 0;
-    while (t >= 0) {
-      const uint32_t v = __hip_atomic_load(status + (size_t)t * 256u + dg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t fl = v >> 30;
-      if (fl == 0u) {
+    bool found = false;
+    while (!found) {
+      uint32_t v[kLookBack];
+#pragma unroll
+      for (uint32_t k = 0; k < kLookBack; ++k)
+        v[k] = t >= (int)k ? __hip_atomic_load(status + (size_t)(t - (int)k) * 256u + dg, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)
+                           : 0u;
+      int taken = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < kLookBack; ++k) {
+        const uint32_t fl = v[k] >> 30;
+        if (fl == 0u) break;  // tile 0 always publishes a prefix, so t - k >= 0 here
+        excl += v[k] & kValMask;
+        ++taken;
+        if (fl == 2u) {
+          found = true;
+          break;
+        }
+      }
+      if (found) break;
+      t -= taken;
+      if (taken == 0) {
         if (++spins > kSpinLimit) {
           atomicOr(err, 1u);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        continue;
       }
-      excl += v & kValMask;
-      if (fl == 2u) break;
-      --t;
     }
     __hip_atomic_store(my_status, kFlagPre | ((excl + cnt) & kValMask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
