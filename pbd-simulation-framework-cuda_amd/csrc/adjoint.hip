@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include "adjoint.hpp"
+#include "adjoint_list.hpp"
 #include "device.hpp"
 
 namespace pbd {
@@ -142,15 +143,9 @@ __device__ __forceinline__ float4 masked(float3 v, uint32_t m) {
 
 __device__ __forceinline__ bool shells_on(const DevParams& P) { return (P.gates & G_FIND_NEIGHBOURS) != 0; }
 
-// Lanes per particle: the neighbour candidates of one particle are dealt to
-// kLanes consecutive lanes (candidate j to lane j mod kLanes within each
-// cell), so that a step of a few ten thousand particles still fills the CUs
-// with waves; the per-lane sums are combined by butterfly shuffles.
-#ifndef PBD_ADJ_LANES
-#define PBD_ADJ_LANES 8
-#endif
-constexpr uint32_t kLanes = PBD_ADJ_LANES;
-static_assert(kLanes >= 1 && kLanes <= 64 && (kLanes & (kLanes - 1)) == 0, "lanes per particle: a power of 2");
+// Lanes per particle (kAdjLanes): a step of a few ten thousand particles
+// still fills the CUs with waves.
+constexpr uint32_t kLanes = kAdjLanes;
 
 __device__ __forceinline__ float lane_sum(float v) {
 #pragma unroll
@@ -204,77 +199,9 @@ __device__ __forceinline__ uint32_t adj_visit(const DevParams& P, uint32_t s, ui
   return cnt;
 }
 
-// Neighbour lists of the reverse passes.  For one set of positions, each lane
-// of a particle keeps the candidates of its share of the 27-cell stencil that
-// lie within the list radius (every pair term is zero beyond it), at most
-// kAdjListMax of them, as j | (27-cell offset << 27).  A particle with a fuller
-// lane is marked kAdjScan on all its lanes, which then scan the stencil.  The
-// passes walk the lists instead of the ~216 candidates, so a wave executes a
-// pair body about 8 instead of 27 times per lane.
-constexpr uint32_t kAdjListMax = 8;
-constexpr uint32_t kAdjScan = 0xFFFFFFFFu;
-struct AdjList {
-  uint32_t* e;    // [(k n + s) kLanes + lane], k < kAdjListMax (nullptr: no lists, scan)
-  uint32_t* cnt;  // [s kLanes + lane]
-};
-
-__device__ __forceinline__ bool lane_any(bool v) {
-  uint32_t x = v ? 1u : 0u;
-#pragma unroll
-  for (uint32_t o = kLanes / 2; o > 0; o >>= 1) x |= (uint32_t)__shfl_xor((int)x, (int)o);
-  return x != 0u;
-}
-
-__global__ void __launch_bounds__(kAdjThreads)
-k_adj_list(DevParams P, const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
-           const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce, AdjList L, uint32_t* __restrict__ flag) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t s = t / kLanes, lane = t % kLanes;
-  if (s >= P.n) return;
-  const uint32_t n = P.n;
-  const float4 qi = Q[s];
-  const CellFrame f = make_frame(P, keys[s]);
-  uint32_t c = 0, cnt = 0;
-  for (int dx = -1; dx <= 1; ++dx) {
-    const uint32_t nx = f.ix + (uint32_t)dx;
-    if (nx >= P.dims[0]) continue;
-    const uint32_t kx = dilated_add(f.kx, dx, kMortonX, 2);
-    const float bx = qi.x + (f.cx - cell_corner(P.origin[0], P.cell, nx));
-    for (int dy = -1; dy <= 1; ++dy) {
-      const uint32_t ny = f.iy + (uint32_t)dy;
-      if (ny >= P.dims[1]) continue;
-      const uint32_t ky = dilated_add(f.ky, dy, kMortonY, 1);
-      const float by = qi.y + (f.cy - cell_corner(P.origin[1], P.cell, ny));
-      for (int dz = -1; dz <= 1; ++dz) {
-        const uint32_t nz = f.iz + (uint32_t)dz;
-        if (nz >= P.dims[2]) continue;
-        const uint32_t kz = dilated_add(f.kz, dz, kMortonZ, 0);
-        const float bz = qi.z + (f.cz - cell_corner(P.origin[2], P.cell, nz));
-        const uint32_t k = kx | ky | kz;
-        const uint32_t st = cs[k];
-        if (st == 0xFFFFFFFFu) continue;
-        const uint32_t en = ce[k];
-        cnt += en - st - ((s >= st && s < en) ? 1u : 0u);
-        const uint32_t off = (uint32_t)((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)) << 27;
-        for (uint32_t j = st + ((lane - st) & (kLanes - 1)); j < en; j += kLanes) {
-          if (j == s) continue;
-          const float4 qj = Q[j];
-          const float rx = bx - qj.x, ry = by - qj.y, rz = bz - qj.z;
-          if (rx * rx + ry * ry + rz * rz < P.list_r2) {
-            if (c < kAdjListMax) L.e[((size_t)c * n + s) * kLanes + lane] = j | off;
-            ++c;
-          }
-        }
-      }
-    }
-  }
-  const bool over = lane_any(c > kAdjListMax);
-  L.cnt[(size_t)s * kLanes + lane] = over ? kAdjScan : c;
-  if (lane == 0 && cnt >= P.cap) flag[0] = 1u;
-}
-
-// adj_visit over the lane's list when there is one (27-cell stencil: every
-// entry is a contact); returns the particle's neighbour count only when it scans.
+// adj_visit over the lane's list when there is one (adjoint_list.hpp; 27-cell
+// stencil: every entry is a contact); returns the particle's neighbour count
+// only when it scans (the list builder checks the cap).
 template <class Body>
 __device__ __forceinline__ uint32_t adj_walk(const DevParams& P, const AdjList& L, uint32_t s, uint32_t lane,
                                              const CellFrame& f, float qx, float qy, float qz,
@@ -635,7 +562,10 @@ hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P0, uint32_t ite
   // neighbour lists for the 27-cell stencil (shells of a wider stencil scan)
   const bool lists = W.LE && n >= kAdjListMinParticles && n < (1u << 27) && !((P.gates & G_FIND_NEIGHBOURS) && P.kw > 1);
   const AdjList L{lists ? W.LE : nullptr, lists ? W.LC : nullptr};
-  if (lists) k_adj_list<<<grid, block, 0, st>>>(P, Qf, T.keys, T.cs, T.ce, L, W.flag);
+  if (lists) {
+    const hipError_t e = launch_adjoint_list_tiles(st, P, W.table, Qf, T.keys, T.cs, T.ce, L, W.flag);
+    if (e != hipSuccess) return e;
+  }
   k_adj_a<<<grid, block, 0, st>>>(P, L, Qf, T.keys, T.cs, T.ce, T.U, T.OM, W.GV, W.GF, W.Ub, W.Xp, W.Eb, W.Ob, W.acc,
                                   W.flag);
   k_adj_b<<<grid, block, 0, st>>>(P, L, Qf, T.keys, T.cs, T.ce, T.OM, W.Eb, W.Ob, W.Xp);
@@ -644,7 +574,10 @@ hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P0, uint32_t ite
   for (uint32_t it = iters; it-- > 0;) {
     const float4* Q = T.Q + (size_t)it * n;
     const float4* QX = fric ? T.QX + (size_t)it * n : nullptr;
-    if (lists) k_adj_list<<<grid, block, 0, st>>>(P, Q, T.keys, T.cs, T.ce, L, W.flag);
+    if (lists) {
+      const hipError_t e = launch_adjoint_list_tiles(st, P, W.table, Q, T.keys, T.cs, T.ce, L, W.flag);
+      if (e != hipSuccess) return e;
+    }
     k_adj_r1<<<grid, block, 0, st>>>(P, L, stab, Q, T.LAM + (size_t)it * n, T.keys, T.cs, T.ce, W.DPb, W.PUb, QX, W.Pn,
                                      W.AB, W.Rb, fric ? W.XF : nullptr, W.acc, W.flag);
     k_adj_r2<<<grid, block, 0, st>>>(P, L, stab, it == 0 ? 1u : 0u, Q, T.keys, T.cs, T.ce, W.AB, W.Rb, W.Pn,
@@ -654,7 +587,7 @@ hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P0, uint32_t ite
   return hipGetLastError();
 }
 
-size_t adjoint_list_entries(uint32_t n) { return (size_t)n * kLanes * kAdjListMax; }
+size_t adjoint_list_entries(uint32_t n) { return (size_t)n * kAdjLanes * kAdjListMax; }
 size_t adjoint_list_counts(uint32_t n) { return (size_t)n * kLanes; }
 
 hipError_t launch_gather_ids(hipStream_t st, uint32_t n, const uint32_t* perm, const uint32_t* ids_in,

@@ -35,6 +35,25 @@ struct AdjTape {
 };
 
 // Adjoint work buffers (n each, sorted order).
+// Lanes per particle in the reverse neighbour passes: the candidates of one
+// particle are dealt to kAdjLanes consecutive lanes (candidate j to lane
+// j mod kAdjLanes within each cell), and the lane sums combined by shuffles.
+#ifndef PBD_ADJ_LANES
+#define PBD_ADJ_LANES 8
+#endif
+constexpr uint32_t kAdjLanes = PBD_ADJ_LANES;
+static_assert(kAdjLanes >= 1 && kAdjLanes <= 64 && (kAdjLanes & (kAdjLanes - 1)) == 0, "lanes per particle: a power of 2");
+
+// Neighbour lists of the reverse passes (built per position set, walked by the
+// passes): per lane at most kAdjListMax entries j | (27-cell offset << 27), or
+// kAdjScan on every lane of a particle whose fullest lane has more.
+constexpr uint32_t kAdjListMax = 8;
+constexpr uint32_t kAdjScan = 0xFFFFFFFFu;
+struct AdjList {
+  uint32_t* e;    // [(k n + s) kAdjLanes + lane], k < kAdjListMax (nullptr: no lists, scan)
+  uint32_t* cnt;  // [s kAdjLanes + lane]
+};
+
 struct AdjWork {
   float4* GX;   // in: d L / d positions out, d L / d velocities out, d L / d external forces out;
   float4* GV;   // out (after the predict pass): the same for the step's inputs, order before the sort
@@ -55,6 +74,7 @@ struct AdjWork {
   float* Rb;    // density adjoint
   double2* acc; // per slot: accumulated d L / d rho0 (x) and d L / d eps (y)
   uint32_t* flag;  // [0]: a neighbour list reached the cap
+  uint32_t table;  // entries of the steps' cell tables
 };
 
 // Reverse of one step: W.GX/GV/GF hold the adjoints of the step's outputs on
@@ -64,6 +84,12 @@ hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P, uint32_t iter
 // The reverse passes walk neighbour lists from this many particles on; below it
 // (64K: 0.81 vs 0.77 ms per step) building them costs what walking them saves.
 constexpr uint32_t kAdjListMinParticles = 1u << 18;
+// The lists of one position set Q (sorted order of the step, keys and cell
+// table cs / ce of `table` entries), built on LDS tiles of 2^5-cell blocks
+// (tiles.hip); flag[0] is set where a particle's 27 cells hold P.cap or more.
+hipError_t launch_adjoint_list_tiles(hipStream_t st, const DevParams& P, uint32_t table, const float4* Q,
+                                     const uint32_t* keys, const uint32_t* cs, const uint32_t* ce, AdjList L,
+                                     uint32_t* flag);
 // Sizes (u32 words) of AdjWork::LE and AdjWork::LC for n particles.
 size_t adjoint_list_entries(uint32_t n);
 size_t adjoint_list_counts(uint32_t n);

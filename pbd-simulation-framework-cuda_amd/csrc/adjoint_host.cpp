@@ -116,6 +116,7 @@ int Fluid::adj_forward(uint32_t frames, uint32_t k_ckpt) {
          (!A.phased || A.alloc(&T.PH, n)) && (!A.friction || A.alloc(&T.QX, (size_t)A.iters * n));
   }
   AdjWork& W = A.W;
+  W.table = A.table;
   ok = ok && A.alloc(&W.GX, n) && A.alloc(&W.GV, n) && A.alloc(&W.GF, n) && A.alloc(&W.Ub, n) && A.alloc(&W.Xp, n) &&
        A.alloc(&W.Eb, n) && A.alloc(&W.Ob, n) && A.alloc(&W.Pb, n) && A.alloc(&W.Xb, n) && A.alloc(&W.DPb, n) &&
        A.alloc(&W.PUb, n) && A.alloc(&W.AB, n) && A.alloc(&W.Pn, n) && A.alloc(&W.Rb, n) && A.alloc(&W.acc, n) &&

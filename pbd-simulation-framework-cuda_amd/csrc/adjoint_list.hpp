@@ -1,0 +1,77 @@
+// adjoint_list.hpp - neighbour lists of the reverse passes (device code; the
+// list layout is declared in adjoint.hpp).
+//
+// For one set of positions, the lanes of a particle hold between them the
+// candidates of its 27-cell stencil that lie within the list radius (every
+// pair term is zero beyond it), at most kAdjListMax per lane, as
+// j | (27-cell offset << 27); a particle that does not fit is marked kAdjScan
+// on all its lanes, which then scan the stencil.  The tile builder (tiles.hip)
+// deals a particle's candidates to its lanes in turn, so that no lane holds
+// more than one entry above another.  adj_list_lane, used for blocks whose halo
+// exceeds the tile capacity, gives lane `lane` the candidates j = lane mod
+// kAdjLanes of each cell, the scan's share.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "adjoint.hpp"
+#include "device.hpp"
+
+namespace pbd {
+
+// lanes of a particle: consecutive lanes of one wave, all active together
+__device__ __forceinline__ bool adj_lane_any(bool v) {
+  uint32_t x = v ? 1u : 0u;
+#pragma unroll
+  for (uint32_t o = kAdjLanes / 2; o > 0; o >>= 1) x |= (uint32_t)__shfl_xor((int)x, (int)o);
+  return x != 0u;
+}
+
+// Lane `lane` of particle s's list, from the global cell table.
+__device__ __forceinline__ void adj_list_lane(const DevParams& P, const float4* __restrict__ Q,
+                                              const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
+                                              const uint32_t* __restrict__ ce, const AdjList& L,
+                                              uint32_t* __restrict__ flag, uint32_t s, uint32_t lane) {
+  const uint32_t n = P.n;
+  const float4 qi = Q[s];
+  const CellFrame f = make_frame(P, keys[s]);
+  uint32_t c = 0, cnt = 0;
+  for (int dx = -1; dx <= 1; ++dx) {
+    const uint32_t nx = f.ix + (uint32_t)dx;
+    if (nx >= P.dims[0]) continue;
+    const uint32_t kx = dilated_add(f.kx, dx, kMortonX, 2);
+    const float bx = qi.x + (f.cx - cell_corner(P.origin[0], P.cell, nx));
+    for (int dy = -1; dy <= 1; ++dy) {
+      const uint32_t ny = f.iy + (uint32_t)dy;
+      if (ny >= P.dims[1]) continue;
+      const uint32_t ky = dilated_add(f.ky, dy, kMortonY, 1);
+      const float by = qi.y + (f.cy - cell_corner(P.origin[1], P.cell, ny));
+      for (int dz = -1; dz <= 1; ++dz) {
+        const uint32_t nz = f.iz + (uint32_t)dz;
+        if (nz >= P.dims[2]) continue;
+        const uint32_t kz = dilated_add(f.kz, dz, kMortonZ, 0);
+        const float bz = qi.z + (f.cz - cell_corner(P.origin[2], P.cell, nz));
+        const uint32_t k = kx | ky | kz;
+        const uint32_t st = cs[k];
+        if (st == 0xFFFFFFFFu) continue;
+        const uint32_t en = ce[k];
+        cnt += en - st - ((s >= st && s < en) ? 1u : 0u);
+        const uint32_t off = (uint32_t)((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)) << 27;
+        for (uint32_t j = st + ((lane - st) & (kAdjLanes - 1)); j < en; j += kAdjLanes) {
+          if (j == s) continue;
+          const float4 qj = Q[j];
+          const float rx = bx - qj.x, ry = by - qj.y, rz = bz - qj.z;
+          if (rx * rx + ry * ry + rz * rz < P.list_r2) {
+            if (c < kAdjListMax) L.e[((size_t)c * n + s) * kAdjLanes + lane] = j | off;
+            ++c;
+          }
+        }
+      }
+    }
+  }
+  const bool over = adj_lane_any(c > kAdjListMax);
+  L.cnt[(size_t)s * kAdjLanes + lane] = over ? kAdjScan : c;
+  if (lane == 0 && cnt >= P.cap) flag[0] = 1u;
+}
+
+}  // namespace pbd

@@ -17,6 +17,7 @@
 // findContacts on) is tiled; wider stencils use the per-particle kernels.
 #include <hip/hip_runtime.h>
 
+#include "adjoint_list.hpp"
 #include "device.hpp"
 #include "kernels.hpp"
 #include "particle_ops.hpp"
@@ -1138,6 +1139,153 @@ hipError_t launch_tile_final(hipStream_t st, int bits, uint32_t grid, bool strid
   }
 #undef PBD_FINAL_S
 #undef PBD_FINAL
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour lists of the reverse passes (adjoint_list.hpp) on tiles.  The
+// halo of a 2^BITS-cell block is copied to LDS once, and one thread per own
+// particle walks its 27 cells there in stencil order (threads of one cell read
+// the same slots; each (x, y) column's three z cells as one slot range),
+// dealing the candidates within the list radius to its lanes in turn: entry c
+// to lane c mod kAdjLanes.  The lanes hold other shares of the
+// candidates than a scan's, so the reverse passes add the same terms in
+// another order.  A particle with more than kAdjLanes x kAdjListMax is marked
+// kAdjScan; a block whose halo exceeds the LDS capacity uses adj_list_lane.
+static_assert(kAdjLanes == 8, "the tile list builder writes rows of 8 entries as two 16-byte stores");
+template <int BITS>
+__device__ __forceinline__ void adjlist_tile(const DevParams& P, uint32_t blk, const float4* __restrict__ Q,
+                                             const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
+                                             const uint32_t* __restrict__ ce, const AdjList& L,
+                                             uint32_t* __restrict__ flag, TileShared<BITS>& sh, float4* s_q) {
+  using G = TileGeom<BITS>;
+  constexpr int CAP = TileCap<BITS>::value;
+  constexpr uint32_t kMax = kAdjLanes * kAdjListMax;
+  int ix0, iy0, iz0;
+  const uint32_t total = tile_setup<BITS, TeamBlock>(P, blk, cs, ce, sh, ix0, iy0, iz0);
+  const uint32_t g0 = sh.g0, g1 = sh.g1;
+  if (total > (uint32_t)CAP) {
+    // whole particles per pass of the loop: a particle's lanes are active together
+    const uint32_t m = (g1 - g0) * kAdjLanes;
+    for (uint32_t u = threadIdx.x; u < m; u += kTileThreads)
+      adj_list_lane(P, Q, keys, cs, ce, L, flag, g0 + u / kAdjLanes, u % kAdjLanes);
+    return;
+  }
+  tile_load<BITS, CAP, TeamBlock>(P, sh, total, Q, ix0, iy0, iz0, s_q, nullptr, 0u, [](uint32_t) { return 0.0f; });
+  __syncthreads();
+  const uint32_t n = P.n;
+  for (uint32_t g = g0 + threadIdx.x; g < g1; g += kTileThreads) {
+    int h;
+    uint32_t slot;
+    tile_locate<BITS>(sh, keys[g], g, ix0, iy0, iz0, h, slot);
+    const float4 qb = s_q[slot];
+    // the last kAdjLanes entries in a shift window w[0..7]; each full row of
+    // the list (entries 8k .. 8k+7, one per lane) is written as 32 bytes
+    uint32_t w[kAdjLanes];
+#pragma unroll
+    for (uint32_t i = 0; i < kAdjLanes; ++i) w[i] = 0u;
+    const auto put_row = [&](uint32_t row) {
+      uint4* dst = reinterpret_cast<uint4*>(L.e + ((size_t)row * n + g) * kAdjLanes);
+      dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+      dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    };
+    uint32_t c = 0;
+    // per (x, y) column the three z cells are one slot range [a0, e), scanned
+    // kScanUnroll slots at a time (reads past e stay inside the LDS array and
+    // are not used); a slot's z offset and sorted index follow from a1, a2
+#pragma unroll
+    for (int dx = -1; dx <= 1; ++dx) {
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int hb = h + dx * G::SX + dy * G::SY;
+        const uint32_t a0 = sh.lstart[hb - 1], a1 = sh.lstart[hb], a2 = sh.lstart[hb + 1], e = sh.lstart[hb + 2];
+        const uint32_t b0 = sh.gstart[hb - 1] - a0, b1 = sh.gstart[hb] - a1, b2 = sh.gstart[hb + 1] - a2;
+        const uint32_t oc = (uint32_t)((dx + 1) * 9 + (dy + 1) * 3);
+        for (uint32_t j0 = a0; j0 < e; j0 += kScanUnroll) {
+          float4 q[kScanUnroll];
+#pragma unroll
+          for (uint32_t u = 0; u < kScanUnroll; ++u) q[u] = s_q[j0 + u];
+#pragma unroll
+          for (uint32_t u = 0; u < kScanUnroll; ++u) {
+            const uint32_t j = j0 + u;
+            const float rx = qb.x - q[u].x, ry = qb.y - q[u].y, rz = qb.z - q[u].z;
+            if (j < e && j != slot && rx * rx + ry * ry + rz * rz < P.list_r2) {
+              const uint32_t z = j < a1 ? 0u : (j < a2 ? 1u : 2u);
+              const uint32_t base = z == 0u ? b0 : (z == 1u ? b1 : b2);  // sorted index of slot j: base + j
+#pragma unroll
+              for (uint32_t i = 0; i + 1 < kAdjLanes; ++i) w[i] = w[i + 1];
+              w[kAdjLanes - 1] = (base + j) | ((oc + z) << 27);
+              ++c;
+              if ((c & (kAdjLanes - 1u)) == 0u && c <= kMax) put_row(c / kAdjLanes - 1u);
+            }
+          }
+        }
+      }
+    }
+    const uint32_t r = c & (kAdjLanes - 1u);
+    if (r != 0u && c < kMax) {  // the last, partial row: its r entries to lanes 0 .. r-1
+      for (uint32_t i = r; i < kAdjLanes; ++i) {
+#pragma unroll
+        for (uint32_t k = 0; k + 1 < kAdjLanes; ++k) w[k] = w[k + 1];
+        w[kAdjLanes - 1] = 0u;
+      }
+      put_row(c / kAdjLanes);
+    }
+    uint32_t cl[kAdjLanes];
+#pragma unroll
+    for (uint32_t lane = 0; lane < kAdjLanes; ++lane) cl[lane] = c > kMax ? kAdjScan : (c + kAdjLanes - 1u - lane) / kAdjLanes;
+    uint4* cdst = reinterpret_cast<uint4*>(L.cnt + (size_t)g * kAdjLanes);
+    cdst[0] = make_uint4(cl[0], cl[1], cl[2], cl[3]);
+    cdst[1] = make_uint4(cl[4], cl[5], cl[6], cl[7]);
+    if (tile_count<BITS>(sh, h) >= P.cap) flag[0] = 1u;
+  }
+}
+
+// Workgroup w takes the blocks w + i gridDim.x, kTileThreads / 2^BITS of them
+// at a time, finds the occupied ones from their own cells' starts and builds
+// those one by one; interleaved so that the occupied blocks, which lie in a
+// few Morton ranges, spread over all workgroups.
+template <int BITS>
+__global__ __launch_bounds__(kTileThreads) void k_tile_adjlist(DevParams P, uint32_t nblocks, uint32_t table,
+                                                               const float4* __restrict__ Q,
+                                                               const uint32_t* __restrict__ keys,
+                                                               const uint32_t* __restrict__ cs,
+                                                               const uint32_t* __restrict__ ce, AdjList L,
+                                                               uint32_t* __restrict__ flag) {
+  __shared__ TileLds<BITS, false> lds;
+  __shared__ uint32_t s_occ;
+  constexpr uint32_t kCells = 1u << BITS;
+  constexpr uint32_t kPer = kTileThreads / kCells;
+  static_assert(kPer <= 32u, "occupancy bits of one batch");
+  for (uint32_t i0 = 0; blockIdx.x + i0 * gridDim.x < nblocks; i0 += kPer) {
+    if (threadIdx.x == 0) s_occ = 0u;
+    __syncthreads();
+    const uint32_t k = threadIdx.x / kCells;
+    const uint32_t b = blockIdx.x + (i0 + k) * gridDim.x;
+    const uint32_t cell = b * kCells + threadIdx.x % kCells;
+    if (b < nblocks && cell < table && cs[cell] != 0xFFFFFFFFu) atomicOr(&s_occ, 1u << k);
+    __syncthreads();
+    const uint32_t occ = s_occ;
+    for (uint32_t i = 0; i < kPer; ++i) {
+      if (!((occ >> i) & 1u)) continue;
+      adjlist_tile<BITS>(P, blockIdx.x + (i0 + i) * gridDim.x, Q, keys, cs, ce, L, flag, lds.big.sh, lds.big.q);
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_adjoint_list_tiles(hipStream_t st, const DevParams& P, uint32_t table, const float4* Q,
+                                     const uint32_t* keys, const uint32_t* cs, const uint32_t* ce, AdjList L,
+                                     uint32_t* flag) {
+  if (P.n == 0 || table == 0) return hipSuccess;
+  constexpr int BITS = 5;
+  const uint32_t nblocks = ((table - 1u) >> BITS) + 1u;
+  const uint32_t per = kTileThreads >> BITS;
+  uint32_t grid = (nblocks + per - 1u) / per;
+  if (grid > 4096u) grid = 4096u;
+  hipLaunchKernelGGL(k_tile_adjlist<BITS>, dim3(grid), dim3(kTileThreads), 0, st, P, nblocks, table, Q, keys, cs, ce,
+                     L, flag);
   return hipGetLastError();
 }
 
