@@ -223,6 +223,7 @@ __device__ __forceinline__ uint32_t adj_walk(const DevParams& P, const AdjList& 
 }
 
 // ---------------------------------------------------------------- A
+template <bool PH>
 __global__ void __launch_bounds__(kAdjThreads)
 k_adj_a(DevParams P, AdjList L, const float4* __restrict__ Q, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
         const uint32_t* __restrict__ ce, const float4* __restrict__ U, const float4* __restrict__ OM,
@@ -236,7 +237,7 @@ k_adj_a(DevParams P, AdjList L, const float4* __restrict__ Q, const uint32_t* __
   const float3 ui = f3(U[s]), gvi = f3(GV[s]), oi = f3(oi4);
   const CellFrame f = make_frame(P, keys[s]);
   const float cv = P.c_visc * P.vol;
-  const uint32_t* __restrict__ ph = P.phs;  // phases: fluid pairs only
+  const uint32_t* __restrict__ ph = PH ? P.phs : nullptr;  // phases: fluid pairs only
   const bool si = ph && ph[s];
   float3 ub = make_float3(0.f, 0.f, 0.f), xb = ub, eta = ub, xs = ub;
   const uint32_t cnt = adj_walk(P, L, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
@@ -270,6 +271,7 @@ k_adj_a(DevParams P, AdjList L, const float4* __restrict__ Q, const uint32_t* __
 }
 
 // ---------------------------------------------------------------- B
+template <bool PH>
 __global__ void __launch_bounds__(kAdjThreads)
 k_adj_b(DevParams P, AdjList L, const float4* __restrict__ Q, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs,
         const uint32_t* __restrict__ ce, const float4* __restrict__ OM, const float4* __restrict__ Eb,
@@ -282,7 +284,7 @@ k_adj_b(DevParams P, AdjList L, const float4* __restrict__ Q, const uint32_t* __
   const CellFrame f = make_frame(P, keys[s]);
   float mb = 0.0f;
   float3 xb = make_float3(0.f, 0.f, 0.f);
-  const uint32_t* __restrict__ ph = P.phs;  // phases: fluid pairs only
+  const uint32_t* __restrict__ ph = PH ? P.phs : nullptr;  // phases: fluid pairs only
   const bool si = ph && ph[s];
   adj_walk(P, L, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
                      if (si || (ph && ph[j])) return;
@@ -304,6 +306,7 @@ k_adj_b(DevParams P, AdjList L, const float4* __restrict__ Q, const uint32_t* __
 }
 
 // ---------------------------------------------------------------- C
+template <bool PH>
 __global__ void __launch_bounds__(kAdjThreads)
 k_adj_c(DevParams P, AdjList L, uint32_t stab, const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
         const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce, const float4* __restrict__ U,
@@ -318,7 +321,7 @@ k_adj_c(DevParams P, AdjList L, uint32_t stab, const float4* __restrict__ Q, con
   const float3 ui = f3(U[s]), obi = f3(Ob[s]);
   const CellFrame f = make_frame(P, key);
   float3 ub = make_float3(0.f, 0.f, 0.f), xb = ub;
-  const uint32_t* __restrict__ ph = P.phs;  // phases: fluid pairs only
+  const uint32_t* __restrict__ ph = PH ? P.phs : nullptr;  // phases: fluid pairs only
   const bool si = ph && ph[s];
   adj_walk(P, L, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
                      if (si || (ph && ph[j])) return;
@@ -343,6 +346,7 @@ k_adj_c(DevParams P, AdjList L, uint32_t stab, const float4* __restrict__ Q, con
 }
 
 // ---------------------------------------------------------------- R1
+template <bool PH>
 __global__ void __launch_bounds__(kAdjThreads)
 k_adj_r1(DevParams P, AdjList L, uint32_t stab, const float4* __restrict__ Q, const float* __restrict__ LAM,
          const uint32_t* __restrict__ keys, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
@@ -361,10 +365,10 @@ k_adj_r1(DevParams P, AdjList L, uint32_t stab, const float4* __restrict__ Q, co
   float3 a = make_float3(0.f, 0.f, 0.f), dsum = a, pn = a, xf = a;
   // phases: density terms between fluid particles only; friction (QX != nullptr)
   // on contacts with a solid, from the displacements p* - x of the iteration
-  const uint32_t* __restrict__ ph = P.phs;
+  const uint32_t* __restrict__ ph = PH ? P.phs : nullptr;
   const bool si = ph && ph[s];
   float3 di = make_float3(0.f, 0.f, 0.f);
-  if (QX) di = f3(qi) - f3(QX[s]);
+  if (PH && QX) di = f3(qi) - f3(QX[s]);
   const uint32_t cnt = adj_walk(
       P, L, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool contact) {
         const float4 qj = Q[j];
@@ -380,7 +384,7 @@ k_adj_r1(DevParams P, AdjList L, uint32_t stab, const float4* __restrict__ Q, co
           pn += kc * (phi * pd - rr * r);
         }
         const bool sj = ph && ph[j];
-        if (QX && touch && (si || sj)) {
+        if (PH && QX && touch && (si || sj)) {
           const float3 u = di - (f3(qj) - f3(QX[j]));
           float3 gr, gu;
           friction_adj(P, r, q.len, q.ir, u, dpi - f3(DPb[j]), gr, gu);
@@ -406,9 +410,9 @@ k_adj_r1(DevParams P, AdjList L, uint32_t stab, const float4* __restrict__ Q, co
   a = lane_sum(a);
   dsum = lane_sum(dsum);
   pn = dpi + lane_sum(pn);
-  if (QX) xf = lane_sum(xf);
+  if (PH && QX) xf = lane_sum(xf);
   if (lane) return;
-  if (QX) XF[s] = make_float4(xf.x, xf.y, xf.z, 0.0f);
+  if (PH && QX) XF[s] = make_float4(xf.x, xf.y, xf.z, 0.0f);
   if (cnt >= P.cap) flag[0] = 1u;
   lb *= P.inv_rho0;
   const float rho = A + P.w0;
@@ -431,6 +435,7 @@ k_adj_r1(DevParams P, AdjList L, uint32_t stab, const float4* __restrict__ Q, co
 // ---------------------------------------------------------------- R2
 // last: the first iteration (it = 0): Pb <- the adjoint of its input
 // positions; otherwise DPb / PUb <- the masked adjoint for iteration it - 1.
+template <bool PH>
 __global__ void __launch_bounds__(kAdjThreads)
 k_adj_r2(DevParams P, AdjList L, uint32_t stab, uint32_t last, const float4* __restrict__ Q, const uint32_t* __restrict__ keys,
          const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce, const float4* __restrict__ AB,
@@ -445,7 +450,7 @@ k_adj_r2(DevParams P, AdjList L, uint32_t stab, uint32_t last, const float4* __r
   const float rbi = Rb[s];
   const CellFrame f = make_frame(P, key);
   float3 pn = make_float3(0.f, 0.f, 0.f);
-  const uint32_t* __restrict__ ph = P.phs;  // phases: fluid pairs only
+  const uint32_t* __restrict__ ph = PH ? P.phs : nullptr;  // phases: fluid pairs only
   const bool si = ph && ph[s];
   adj_walk(P, L, s, lane, f, qi.x, qi.y, qi.z, cs, ce, [&](uint32_t j, float bx, float by, float bz, bool) {
                      if (si || (ph && ph[j])) return;
@@ -460,7 +465,7 @@ k_adj_r2(DevParams P, AdjList L, uint32_t stab, uint32_t last, const float4* __r
   if (lane) return;
   // friction reads x of this iteration: its adjoint joins x's before it
   float4 xv = Xb[s];
-  if (XF) {
+  if (PH && XF) {
     const float4 xf = XF[s];
     xv = make_float4(xv.x + xf.x, xv.y + xf.y, xv.z + xf.z, 0.0f);
     Xb[s] = xv;
@@ -566,11 +571,17 @@ hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P0, uint32_t ite
     const hipError_t e = launch_adjoint_list_tiles(st, P, W.table, Qf, T.keys, T.cs, T.ce, L, W.flag);
     if (e != hipSuccess) return e;
   }
-  k_adj_a<<<grid, block, 0, st>>>(P, L, Qf, T.keys, T.cs, T.ce, T.U, T.OM, W.GV, W.GF, W.Ub, W.Xp, W.Eb, W.Ob, W.acc,
-                                  W.flag);
-  k_adj_b<<<grid, block, 0, st>>>(P, L, Qf, T.keys, T.cs, T.ce, T.OM, W.Eb, W.Ob, W.Xp);
-  k_adj_c<<<grid, block, 0, st>>>(P, L, stab, Qf, T.keys, T.cs, T.ce, T.U, T.OM, W.Ob, W.Ub, W.Xp, W.GX, W.Xb, W.DPb,
-                                  W.PUb, W.acc);
+  // the phase masks and friction are compiled only into the kernels of phased steps
+  const bool phased = T.PH != nullptr;
+#define PBD_ADJ_LAUNCH(K, ...)                                  \
+  do {                                                          \
+    if (phased) K<true><<<grid, block, 0, st>>>(__VA_ARGS__);   \
+    else K<false><<<grid, block, 0, st>>>(__VA_ARGS__);         \
+  } while (0)
+  PBD_ADJ_LAUNCH(k_adj_a, P, L, Qf, T.keys, T.cs, T.ce, T.U, T.OM, W.GV, W.GF, W.Ub, W.Xp, W.Eb, W.Ob, W.acc, W.flag);
+  PBD_ADJ_LAUNCH(k_adj_b, P, L, Qf, T.keys, T.cs, T.ce, T.OM, W.Eb, W.Ob, W.Xp);
+  PBD_ADJ_LAUNCH(k_adj_c, P, L, stab, Qf, T.keys, T.cs, T.ce, T.U, T.OM, W.Ob, W.Ub, W.Xp, W.GX, W.Xb, W.DPb, W.PUb,
+                 W.acc);
   for (uint32_t it = iters; it-- > 0;) {
     const float4* Q = T.Q + (size_t)it * n;
     const float4* QX = fric ? T.QX + (size_t)it * n : nullptr;
@@ -578,11 +589,12 @@ hipError_t launch_adjoint_step(hipStream_t st, const DevParams& P0, uint32_t ite
       const hipError_t e = launch_adjoint_list_tiles(st, P, W.table, Q, T.keys, T.cs, T.ce, L, W.flag);
       if (e != hipSuccess) return e;
     }
-    k_adj_r1<<<grid, block, 0, st>>>(P, L, stab, Q, T.LAM + (size_t)it * n, T.keys, T.cs, T.ce, W.DPb, W.PUb, QX, W.Pn,
-                                     W.AB, W.Rb, fric ? W.XF : nullptr, W.acc, W.flag);
-    k_adj_r2<<<grid, block, 0, st>>>(P, L, stab, it == 0 ? 1u : 0u, Q, T.keys, T.cs, T.ce, W.AB, W.Rb, W.Pn,
-                                     fric ? W.XF : nullptr, W.Xb, W.DPb, W.PUb, W.Pb);
+    PBD_ADJ_LAUNCH(k_adj_r1, P, L, stab, Q, T.LAM + (size_t)it * n, T.keys, T.cs, T.ce, W.DPb, W.PUb, QX, W.Pn, W.AB,
+                   W.Rb, fric ? W.XF : nullptr, W.acc, W.flag);
+    PBD_ADJ_LAUNCH(k_adj_r2, P, L, stab, it == 0 ? 1u : 0u, Q, T.keys, T.cs, T.ce, W.AB, W.Rb, W.Pn,
+                   fric ? W.XF : nullptr, W.Xb, W.DPb, W.PUb, W.Pb);
   }
+#undef PBD_ADJ_LAUNCH
   k_adj_predict<<<grid1, block, 0, st>>>(P, T.perm, T.Xin, T.Vin, T.Fin, W.Pb, W.Xb, W.GX, W.GV, W.GF);
   return hipGetLastError();
 }

@@ -1281,9 +1281,8 @@ hipError_t launch_adjoint_list_tiles(hipStream_t st, const DevParams& P, uint32_
   if (P.n == 0 || table == 0) return hipSuccess;
   constexpr int BITS = 5;
   const uint32_t nblocks = ((table - 1u) >> BITS) + 1u;
-  const uint32_t per = kTileThreads >> BITS;
-  uint32_t grid = (nblocks + per - 1u) / per;
-  if (grid > 4096u) grid = 4096u;
+  // a workgroup per block up to 4096 of them (small scenes still fill the CUs)
+  const uint32_t grid = nblocks < 4096u ? nblocks : 4096u;
   hipLaunchKernelGGL(k_tile_adjlist<BITS>, dim3(grid), dim3(kTileThreads), 0, st, P, nblocks, table, Q, keys, cs, ce,
                      L, flag);
   return hipGetLastError();
