@@ -327,7 +327,9 @@ int pbd_adj_ids(pbd_ctx* ctx, uint32_t* initial_index, size_t count);
  * k_ckpt frames is recomputed from its checkpoint while its steps are recorded
  * (about 104 + 20 x solverIterations bytes per particle and step; phases add 4,
  * friction 16 x solverIterations), then the
- * reverse kernels run over the same sorted neighbour structure.  The context
+ * reverse kernels run over the same sorted neighbour structure (from 2^18
+ * particles on with neighbour lists rebuilt per position set: 288 B per
+ * particle, allocated by pbd_adj_forward).  The context
  * is left as pbd_adj_forward left it.  PBD_ERR_UNSUPPORTED if a neighbour list
  * reached maxNeighboursPerParticle (a capped neighbour relation is not
  * symmetric). */

@@ -526,12 +526,17 @@ __global__ void k_gather_ids(uint32_t n, const uint32_t* __restrict__ perm, cons
 }
 
 constexpr uint32_t kReduceThreads = 1024;
+// Workgroup b sums chunk b of the m values (chunks of ceil(m / gridDim.x)) and
+// writes part[b]: a fixed order, so the result does not vary between runs.
 __global__ void __launch_bounds__(kReduceThreads)
-k_adj_reduce(uint32_t n, const double2* __restrict__ acc, double* __restrict__ out) {
+k_adj_reduce(uint32_t m, const double2* __restrict__ in, double2* __restrict__ part) {
   __shared__ double sx[kReduceThreads], sy[kReduceThreads];
+  const uint32_t chunk = (m + gridDim.x - 1u) / gridDim.x;
+  const uint32_t b = blockIdx.x * chunk;
+  const uint32_t e = b + chunk < m ? b + chunk : m;
   double x = 0.0, y = 0.0;
-  for (uint32_t i = threadIdx.x; i < n; i += kReduceThreads) {
-    const double2 a = acc[i];
+  for (uint32_t i = b + threadIdx.x; i < e; i += kReduceThreads) {
+    const double2 a = in[i];
     x += a.x;
     y += a.y;
   }
@@ -545,10 +550,7 @@ k_adj_reduce(uint32_t n, const double2* __restrict__ acc, double* __restrict__ o
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    out[0] = sx[0];
-    out[1] = sy[0];
-  }
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(sx[0], sy[0]);
 }
 
 }  // namespace
@@ -610,7 +612,13 @@ hipError_t launch_gather_ids(hipStream_t st, uint32_t n, const uint32_t* perm, c
 }
 
 hipError_t launch_adjoint_reduce(hipStream_t st, uint32_t n, const double2* acc, double* out) {
-  k_adj_reduce<<<1, kReduceThreads, 0, st>>>(n, acc, out);
+  double2* res = reinterpret_cast<double2*>(out);
+  double2* part = res + 1;
+  uint32_t parts = (n + kReduceThreads - 1u) / kReduceThreads;
+  if (parts < 1u) parts = 1u;
+  if (parts > kAdjReduceParts) parts = kAdjReduceParts;
+  k_adj_reduce<<<parts, kReduceThreads, 0, st>>>(n, acc, part);
+  k_adj_reduce<<<1, kReduceThreads, 0, st>>>(parts, part, res);
   return hipGetLastError();
 }
 

@@ -96,7 +96,11 @@ size_t adjoint_list_counts(uint32_t n);
 // ids_out[s] = ids_in[perm[s]] (identity of the particles through a sort)
 hipError_t launch_gather_ids(hipStream_t st, uint32_t n, const uint32_t* perm, const uint32_t* ids_in,
                              uint32_t* ids_out);
-// out = sum over slots of W.acc (one block; out in device memory)
+// out[0..1] = sum over slots of W.acc, in a fixed order: kAdjReduceParts
+// contiguous chunks, then their partial sums.  out (device memory) holds
+// kAdjReduceWords doubles: the result, then the partial sums.
+constexpr uint32_t kAdjReduceParts = 256;
+constexpr uint32_t kAdjReduceWords = 2 + 2 * kAdjReduceParts;
 hipError_t launch_adjoint_reduce(hipStream_t st, uint32_t n, const double2* acc, double* out);
 
 }  // namespace pbd

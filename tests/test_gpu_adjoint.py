@@ -302,6 +302,70 @@ def test_gpu_adjoint_neighbour_lists_262k(F, O):
         assert _rel(g[k], go[k]) < 1e-2
 
 
+def _max_halo(x, cell, dims):
+    """Largest particle count over the halos (4 x 6 x 6 cells) of the aligned
+    2 x 4 x 4-cell blocks that hold the positions x."""
+    ijk = np.clip(np.floor(x / cell).astype(np.int64), 0, np.array(dims) - 1)
+    c = np.zeros((dims[0] + 4, dims[1] + 8, dims[2] + 8), np.int64)
+    np.add.at(c, (ijk[:, 0] + 1, ijk[:, 1] + 1, ijk[:, 2] + 1), 1)
+    s = np.zeros((c.shape[0] + 1, c.shape[1] + 1, c.shape[2] + 1), np.int64)
+    s[1:, 1:, 1:] = c.cumsum(0).cumsum(1).cumsum(2)
+    bx, by, bz = np.meshgrid(np.arange(0, dims[0], 2), np.arange(0, dims[1], 4), np.arange(0, dims[2], 4),
+                             indexing="ij")
+    x0, x1, y0, y1, z0, z1 = bx, bx + 4, by, by + 6, bz, bz + 6
+    h = (s[x1, y1, z1] - s[x0, y1, z1] - s[x1, y0, z1] - s[x1, y1, z0] + s[x0, y0, z1] + s[x0, y1, z0]
+         + s[x1, y0, z0] - s[x0, y0, z0])
+    return int(h.max())
+
+
+def test_gpu_adjoint_neighbour_lists_dense_blocks(F, O):
+    """Neighbour lists where the blocks' halos exceed the LDS tile: 2^18
+    particles at their own rest density, a 64^3 lattice at spacing 0.04 with
+    particle diameter 0.04 and spiky width 2.5 (h = cell = 0.1: about 16
+    particles per cell, at most 8 per axis in 3 cells, so under the neighbour
+    cap of 512), so that the lists come from the per-particle builder; one
+    frame against the FP64 oracle's reverse mode."""
+    p = F.load_params(F.config_path("dam_break_1m"))
+    p.box_width = p.box_height = p.box_depth = 64
+    p.particle_radius = 0.02
+    p.box_spacing = 0.04
+    p.kernel_width_spiky = 2.5
+    pos, _, _ = F.generate_box(p)
+    x0 = pos[:, :3].astype(np.float32)
+    n = x0.shape[0]
+    assert n == 1 << 18
+    d = O.derive(p)
+    assert abs(d.h_spiky - 0.1) < 1e-6 and abs(d.h_poly - 0.1) < 1e-6
+    halo = _max_halo(x0.astype(np.float64), d.h_spiky, [int(v) for v in p.grid_dims])
+    assert halo > 1920, halo  # the blocks exceed the tile capacity
+    rng = np.random.default_rng(9)
+    v0 = rng.normal(0.0, 0.1, x0.shape).astype(np.float32)
+    nsub = p.substeps
+    flush(f"neighbour lists, dense blocks: n={n} largest halo {halo} steps={nsub}")
+    x64, v64 = x0.astype(np.float64), v0.astype(np.float64)
+    X, _, _, og = O.forward64(p, x64, v64, nsub=nsub)
+    target = X[np.argsort(og)] + rng.normal(0.0, 0.01, x0.shape)
+    g, ids, xT = _gpu_grad(F, p, x0, v0, 1, 1, target)
+    ex = np.abs(X[np.argsort(og)] - xT[np.argsort(ids)]).max(axis=1)
+    flush(f"  forward: max {ex.max():.2e}, 99.9th pct {np.quantile(ex, 0.999):.2e}")
+    assert np.quantile(ex, 0.999) < 1e-4 and ex.max() < 1e-3
+    gxo = 2.0 * (X - target[og]) / n
+    go = O.gradient64(p, x64, v64, gxo, np.zeros_like(gxo), nsub=nsub)
+    # the bound of the 262K test: 1e-3 without the worst 0.3 % (contacts that
+    # take other branches in FP32 than in FP64), 1e-2 over all particles
+    drop = int(0.003 * n)
+    for k in ("v0", "x0"):
+        dk = g[k] - go[k]
+        keep = np.argsort(np.linalg.norm(dk, axis=1))[: n - drop]
+        err = np.linalg.norm(dk) / np.linalg.norm(go[k])
+        trim = np.linalg.norm(dk[keep]) / np.linalg.norm(go[k][keep])
+        flush(f"  {k}: relative L2 error {err:.2e}, without the worst {drop} particles {trim:.2e}")
+        assert trim < 1e-3 and err < 1e-2
+    for k in ("rho0", "eps"):
+        flush(f"  {k}: gpu {g[k]:.6e} oracle {go[k]:.6e}")
+        assert _rel(g[k], go[k]) < 1e-2
+
+
 def test_gpu_adjoint_with_phases_checkpoints_are_exact(F, O):
     """Checkpoints keep the phases in the state's order: recomputed segments
     replay the phased forward bit for bit."""
