@@ -584,6 +584,35 @@ def test_gpu_full_size_properties(F):
     assert (x[:, :3] >= lo - 1e-5).all() and (x[:, :3] <= hi + 1e-5).all()
 
 
+def test_gpu_sort_16m_properties(F):
+    """16,777,216 particles (dam_break_16m): the digit histogram takes several
+    4096-key tiles per workgroup and adds into 32 histogram copies.  After a
+    step the keys are sorted, the ids are a permutation, equal keys keep their
+    input order (the ids of a run increase: the sort is stable), and the cell
+    ranges match the keys."""
+    p = F.load_params(F.config_path("dam_break_16m"))
+    flush("[sort16m] creating 16.7M context")
+    ctx = F.Context(p)
+    n = ctx.n
+    assert n == 1 << 24
+    ctx.step(1)
+    ctx.sync()
+    keys = ctx.download("cell_ids").astype(np.int64)
+    ids = ctx.download("particle_ids").astype(np.int64)
+    st_ = ctx.download("cell_starts")
+    en_ = ctx.download("cell_endings")
+    ctx.close()
+    flush("[sort16m] checking")
+    d = np.diff(keys)
+    assert np.all(d >= 0)
+    assert np.all(np.diff(ids)[d == 0] > 0)
+    assert np.array_equal(np.sort(ids), np.arange(n))
+    occ = np.unique(keys)
+    first = np.searchsorted(keys, occ, "left")
+    last = np.searchsorted(keys, occ, "right")
+    assert np.array_equal(st_[occ], first) and np.array_equal(en_[occ], last)
+
+
 def test_gpu_frame_output_and_headless_driver(F, O, tmp_path):
     """pbd_write_frame writes the context's positions4 / colors4 in buffer order;
     pbd_run (the headless engine loop) steps a config and writes its frames."""
