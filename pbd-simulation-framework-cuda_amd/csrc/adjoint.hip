@@ -210,8 +210,14 @@ __device__ __forceinline__ uint32_t adj_walk(const DevParams& P, const AdjList& 
   const uint32_t c = L.e ? L.cnt[(size_t)s * kLanes + lane] : kAdjScan;
   if (c == kAdjScan) return adj_visit(P, s, lane, f, qx, qy, qz, cs, ce, body);
   const uint32_t n = P.n;
-  for (uint32_t k = 0; k < c; ++k) {
-    const uint32_t e = L.e[((size_t)k * n + s) * kLanes + lane];
+  // the lane's entries, all loads in flight before the first pair
+  uint32_t ev[kAdjListMax];
+#pragma unroll
+  for (uint32_t k = 0; k < kAdjListMax; ++k) ev[k] = k < c ? L.e[((size_t)k * n + s) * kLanes + lane] : 0u;
+#pragma unroll
+  for (uint32_t k = 0; k < kAdjListMax; ++k) {
+    if (k >= c) break;
+    const uint32_t e = ev[k];
     const uint32_t j = e & 0x07FFFFFFu, off = e >> 27;
     const int dx = (int)(off / 9u) - 1, dy = (int)(off / 3u % 3u) - 1, dz = (int)(off % 3u) - 1;
     const float bx = qx + (f.cx - cell_corner(P.origin[0], P.cell, f.ix + (uint32_t)dx));
